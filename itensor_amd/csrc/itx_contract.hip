@@ -1,0 +1,720 @@
+// Contraction engine of libitx: dense and block-sparse (QN) tensor
+// contractions as one persistent MFMA GEMM kernel with the index
+// permutations folded into offset-table driven tile loads and stores.
+//
+// Reference behaviour replaced:
+//   contract() / CProps (tensor/contract.cc:137-866) reduces a binary
+//   contraction to permute(A), permute(B) -> dgemm -> permute(C).  Here the
+//   permutations never touch memory: the plan records, for every logical
+//   row m, column n and contracted position k of the GEMM, the element
+//   offset inside each operand (tables aM,aK,bK,bN,cM,cN), so the kernel
+//   gathers tiles straight from the original layouts.
+//   doTask(Contract,QDense,QDense) (itdata/qdense.cc:671-743) runs one
+//   contract() per (A block, B block) pair; here all pairs of one
+//   contraction are tasks of a single launch, grouped by C block so that
+//   all pairs feeding one C tile are summed in registers (the reference's
+//   OpenMP path groups by output block for the same reason,
+//   itdata/qutil.h:271-349).
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <sstream>
+
+#include "itx_internal.h"
+
+namespace itx {
+namespace {
+
+constexpr int NT = 256;  // threads per workgroup (4 wave64)
+constexpr int BK = 16;   // K depth of one LDS stage
+
+template <typename T>
+struct MF;
+template <>
+struct MF<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // swizzle of LDS columns per K row (keeps 8-byte accesses conflict free)
+  static __device__ __forceinline__ int sw(int k) { return (k >> 1) & 7; }
+  // output row of accumulator element r held by `lane` (measured on gfx950:
+  // the f64 16x16x4 result interleaves rows, row = lane/16 + 4r)
+  static __device__ __forceinline__ int orow(int lane, int r) { return (lane >> 4) + 4 * r; }
+  template <int BMX>
+  static constexpr int pad() { return 16; }
+};
+template <>
+struct MF<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int sw(int k) { return (k >> 2) & 3; }
+  // f32 16x16x4 result: blocked rows, row = 4*(lane/16) + r
+  static __device__ __forceinline__ int orow(int lane, int r) { return 4 * (lane >> 4) + r; }
+  template <int BMX>
+  static constexpr int pad() { return BMX == 32 ? 48 : 16; }
+};
+
+// Loader of one operand tile (rows x BK) into LDS laid out [k][row].
+// KC: the operand is contiguous along k (lanes walk k), else along rows.
+template <typename T, int BR, bool KC>
+struct TileLoader {
+  static constexpr int PER = BR * BK / NT;  // elements per thread
+  static constexpr int LD = BR + MF<T>::template pad<BR>();
+  double r[PER];
+  int64_t roff[KC ? PER : 1];
+  bool rv[KC ? PER : 1];
+  int lr, lk;  // thread's base row / base k
+
+  __device__ __forceinline__ void init(int tid, int row0, int R, const int64_t *__restrict__ tabR) {
+    if constexpr (KC) {
+      lk = tid % BK;
+      lr = tid / BK;
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        int row = row0 + lr + (NT / BK) * p;
+        rv[p] = row < R;
+        roff[p] = rv[p] ? tabR[row] : 0;
+      }
+    } else {
+      lr = tid % BR;
+      lk = tid / BR;
+      int row = row0 + lr;
+      rv[0] = row < R;
+      roff[0] = rv[0] ? tabR[row] : 0;
+    }
+  }
+  __device__ __forceinline__ void load(const double *__restrict__ base, int k0, int K,
+                                       const int64_t *__restrict__ tabK) {
+    if constexpr (KC) {
+      int k = k0 + lk;
+      bool kv = k < K;
+      int64_t ko = kv ? tabK[k] : 0;
+#pragma unroll
+      for (int p = 0; p < PER; ++p) r[p] = (kv && rv[p]) ? base[roff[p] + ko] : 0.0;
+    } else {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        int k = k0 + lk + (NT / BR) * p;
+        bool kv = k < K;
+        int64_t ko = kv ? tabK[k] : 0;
+        r[p] = (kv && rv[0]) ? base[roff[0] + ko] : 0.0;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(T *__restrict__ s) const {
+    if constexpr (KC) {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        int row = lr + (NT / BK) * p;
+        s[lk * LD + (row ^ MF<T>::sw(lk))] = T(r[p]);
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        int k = lk + (NT / BR) * p;
+        s[k * LD + (lr ^ MF<T>::sw(k))] = T(r[p]);
+      }
+    }
+  }
+};
+
+template <typename T, int BM, int BN, bool AKC, bool BKC>
+__global__ __launch_bounds__(NT) void k_gemm(const double *__restrict__ A, const double *__restrict__ B,
+                                             double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                             const WorkUnit *__restrict__ units, int nunits,
+                                             const int64_t *__restrict__ tab, double alpha, double beta) {
+  using v4 = typename MF<T>::v4;
+  using LA = TileLoader<T, BM, AKC>;
+  using LB = TileLoader<T, BN, BKC>;
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  __shared__ T As[2][BK * LA::LD];
+  __shared__ T Bs[2][BK * LB::LD];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // Units are spread so that each XCD (blocks b with equal b % 8 land on
+  // the same XCD) walks a contiguous chunk of the unit list: neighbouring
+  // tiles share A rows / B columns, which then stay in that XCD's L2.
+  const int G = gridDim.x, b = blockIdx.x;
+  int ustart, uend, ustep;
+  if (G % 8 == 0 && nunits >= G) {
+    const int nb = G / 8, x = b % 8, lb = b / 8;
+    const int chunk = (nunits + 7) / 8;
+    ustart = x * chunk + lb;
+    uend = min(nunits, (x + 1) * chunk);
+    ustep = nb;
+  } else {
+    ustart = b;
+    uend = nunits;
+    ustep = G;
+  }
+
+  for (int u = ustart; u < uend; u += ustep) {
+    const WorkUnit wu = units[u];
+    v4 acc[FN][FM];
+#pragma unroll
+    for (int i = 0; i < FN; ++i)
+#pragma unroll
+      for (int j = 0; j < FM; ++j) acc[i][j] = v4{0, 0, 0, 0};
+
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      if (tk.K <= 0) continue;
+      LA la;
+      LB lb;
+      la.init(tid, wu.m0, tk.M, tab + tk.tAM);
+      lb.init(tid, wu.n0, tk.N, tab + tk.tBN);
+      const double *Ab = A + tk.a_off;
+      const double *Bb = B + tk.b_off;
+      const int64_t *tAK = tab + tk.tAK;
+      const int64_t *tBK = tab + tk.tBK;
+      la.load(Ab, 0, tk.K, tAK);
+      lb.load(Bb, 0, tk.K, tBK);
+      la.store(As[0]);
+      lb.store(Bs[0]);
+      __syncthreads();
+      int cur = 0;
+      for (int k0 = 0; k0 < tk.K; k0 += BK) {
+        const bool nxt = k0 + BK < tk.K;
+        if (nxt) {
+          la.load(Ab, k0 + BK, tk.K, tAK);
+          lb.load(Bb, k0 + BK, tk.K, tBK);
+        }
+        const T *as = As[cur];
+        const T *bs = Bs[cur];
+#pragma unroll
+        for (int kq = 0; kq < BK / 4; ++kq) {
+          const int kr = kq * 4 + (lane >> 4);
+          T af[FM], bf[FN];
+#pragma unroll
+          for (int j = 0; j < FM; ++j) {
+            int m = wm * WM + j * 16 + (lane & 15);
+            af[j] = as[kr * LA::LD + (m ^ MF<T>::sw(kr))];
+          }
+#pragma unroll
+          for (int i = 0; i < FN; ++i) {
+            int n = wn * WN + i * 16 + (lane & 15);
+            bf[i] = bs[kr * LB::LD + (n ^ MF<T>::sw(kr))];
+          }
+          // D^T tile: MFMA rows run over n, columns over m, so that the
+          // 16 lanes of a row hold 16 consecutive m (coalesced C stores).
+#pragma unroll
+          for (int i = 0; i < FN; ++i)
+#pragma unroll
+            for (int j = 0; j < FM; ++j) acc[i][j] = MF<T>::mma(bf[i], af[j], acc[i][j]);
+        }
+        if (nxt) {
+          la.store(As[cur ^ 1]);
+          lb.store(Bs[cur ^ 1]);
+        }
+        __syncthreads();
+        cur ^= 1;
+      }
+    }
+
+    // epilogue: C = alpha*acc + beta*C through the C offset tables
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int64_t *tCM = tab + t0.tCM;
+    const int64_t *tCN = tab + t0.tCN;
+    double *Cb = C + t0.c_off;
+#pragma unroll
+    for (int j = 0; j < FM; ++j) {
+      const int m = wu.m0 + wm * WM + j * 16 + (lane & 15);
+      if (m >= t0.M) continue;
+      const int64_t cm = tCM[m];
+#pragma unroll
+      for (int i = 0; i < FN; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = wu.n0 + wn * WN + i * 16 + MF<T>::orow(lane, r);
+          if (n < t0.N) {
+            const int64_t ad = cm + tCN[n];
+            double v = alpha * double(acc[i][j][r]);
+            if (beta != 0.0) v += beta * Cb[ad];
+            Cb[ad] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---- host side --------------------------------------------------------
+
+struct Digit {
+  int64_t dim, s1, s2;
+};
+
+// Remove unit digits and fuse neighbours that are contiguous in both
+// operands (the same simplification CProps makes before its GEMM).
+void simplify(std::vector<Digit> &d) {
+  std::vector<Digit> o;
+  for (auto &x : d) {
+    if (x.dim == 1) continue;
+    if (!o.empty() && o.back().s1 * o.back().dim == x.s1 && o.back().s2 * o.back().dim == x.s2) {
+      o.back().dim *= x.dim;
+    } else {
+      o.push_back(x);
+    }
+  }
+  d.swap(o);
+}
+
+int64_t digits_size(const std::vector<Digit> &d) {
+  int64_t s = 1;
+  for (auto &x : d) s *= x.dim;
+  return s;
+}
+
+// Append the two offset tables of a digit set to tab; returns the start of
+// the first table (the second starts size() later).
+int64_t emit_tables(const std::vector<Digit> &d, std::vector<int64_t> &tab) {
+  const int64_t n = digits_size(d);
+  const int64_t start = int64_t(tab.size());
+  tab.resize(tab.size() + 2 * size_t(n));
+  int64_t *t1 = tab.data() + start, *t2 = t1 + n;
+  const int r = int(d.size());
+  std::vector<int64_t> ctr(r, 0);
+  int64_t o1 = 0, o2 = 0;
+  for (int64_t x = 0; x < n; ++x) {
+    t1[x] = o1;
+    t2[x] = o2;
+    for (int i = 0; i < r; ++i) {  // first digit fastest
+      ctr[i]++;
+      o1 += d[i].s1;
+      o2 += d[i].s2;
+      if (ctr[i] < d[i].dim) break;
+      o1 -= d[i].s1 * d[i].dim;
+      o2 -= d[i].s2 * d[i].dim;
+      ctr[i] = 0;
+    }
+  }
+  return start;
+}
+
+struct LabelMap {
+  // for A index i: contracted partner in B (or -1) / position in C (or -1)
+  std::vector<int> a2b, a2c, b2a, b2c;
+};
+
+int map_labels(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t *labB, int rC,
+               const int32_t *labC, LabelMap &lm) {
+  if (rA < 0 || rB < 0 || rC < 0 || rA > ITX_MAX_RANK || rB > ITX_MAX_RANK || rC > ITX_MAX_RANK)
+    return set_err(ctx, ITX_ERR_ARG, "contraction: rank out of range");
+  lm.a2b.assign(rA, -1);
+  lm.a2c.assign(rA, -1);
+  lm.b2a.assign(rB, -1);
+  lm.b2c.assign(rB, -1);
+  std::vector<int> cused(rC, 0);
+  for (int i = 0; i < rA; ++i) {
+    if (labA[i] < 0) {
+      for (int j = 0; j < rB; ++j)
+        if (labB[j] == labA[i]) {
+          if (lm.a2b[i] >= 0) return set_err(ctx, ITX_ERR_ARG, "contraction: repeated label in B");
+          lm.a2b[i] = j;
+          lm.b2a[j] = i;
+        }
+      if (lm.a2b[i] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: contracted label of A missing in B");
+    } else {
+      for (int c = 0; c < rC; ++c)
+        if (labC[c] == labA[i]) {
+          if (lm.a2c[i] >= 0) return set_err(ctx, ITX_ERR_ARG, "contraction: repeated label in C");
+          lm.a2c[i] = c;
+          cused[c]++;
+        }
+      if (lm.a2c[i] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: label of A missing in C");
+    }
+  }
+  for (int j = 0; j < rB; ++j) {
+    if (labB[j] < 0) {
+      if (lm.b2a[j] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: contracted label of B missing in A");
+    } else {
+      for (int c = 0; c < rC; ++c)
+        if (labC[c] == labB[j]) {
+          if (lm.b2c[j] >= 0) return set_err(ctx, ITX_ERR_ARG, "contraction: repeated label in C");
+          lm.b2c[j] = c;
+          cused[c]++;
+        }
+      if (lm.b2c[j] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: label of B missing in C");
+    }
+  }
+  for (int c = 0; c < rC; ++c)
+    if (cused[c] != 1) return set_err(ctx, ITX_ERR_ARG, "contraction: label of C not found exactly once in A,B");
+  return ITX_OK;
+}
+
+void colmajor_strides(int r, const int64_t *d, int64_t *s) {
+  int64_t x = 1;
+  for (int i = 0; i < r; ++i) {
+    s[i] = x;
+    x *= d[i];
+  }
+}
+
+// Tile shapes of the engine (square tiles; index stored in the plan).
+constexpr int kTiles[3] = {128, 64, 32};
+
+template <typename T, int BM, bool AKC, bool BKC>
+void launch_t(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C, double alpha,
+              double beta) {
+  hipLaunchKernelGGL((k_gemm<T, BM, BM, AKC, BKC>), dim3(p->grid), dim3(NT), 0, s, A, B, C, p->d_tasks,
+                     p->d_units, p->nunits, p->d_tab, alpha, beta);
+}
+
+template <typename T, int BM>
+void launch_bm(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C, double alpha,
+               double beta) {
+  if (p->a_kcontig) {
+    if (p->b_kcontig)
+      launch_t<T, BM, true, true>(p, s, A, B, C, alpha, beta);
+    else
+      launch_t<T, BM, true, false>(p, s, A, B, C, alpha, beta);
+  } else {
+    if (p->b_kcontig)
+      launch_t<T, BM, false, true>(p, s, A, B, C, alpha, beta);
+    else
+      launch_t<T, BM, false, false>(p, s, A, B, C, alpha, beta);
+  }
+}
+
+template <typename T>
+void launch_prec(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C,
+                 double alpha, double beta) {
+  switch (kTiles[p->tile]) {
+    case 128: launch_bm<T, 128>(p, s, A, B, C, alpha, beta); break;
+    case 64: launch_bm<T, 64>(p, s, A, B, C, alpha, beta); break;
+    default: launch_bm<T, 32>(p, s, A, B, C, alpha, beta); break;
+  }
+}
+
+}  // namespace
+
+int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t *labB, int rC,
+               const int32_t *labC, int ntasks, const int64_t *offs, const int64_t *blkA,
+               const int64_t *blkB, const int64_t *blkC, int prec, itx_plan **out) {
+  *out = nullptr;
+  if (prec != ITX_PREC_F64 && prec != ITX_PREC_F32)
+    return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: precision form not built");
+  if (ntasks < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: negative task count");
+  LabelMap lm;
+  int rc = map_labels(ctx, rA, labA, rB, labB, rC, labC, lm);
+  if (rc) return rc;
+
+  std::vector<GemmTask> tasks;
+  std::vector<int64_t> tab;
+  tasks.reserve(ntasks);
+  // A is read along k if its unit-stride digit is a contracted one (else
+  // along m); likewise B.  Decided on the first non-empty task.
+  bool akc = false, bkc = false, kc_set = false;
+
+  double flops = 0;
+  for (int t = 0; t < ntasks; ++t) {
+    const int64_t *dA = blkA + size_t(t) * rA;
+    const int64_t *dB = blkB + size_t(t) * rB;
+    int64_t dC[ITX_MAX_RANK], sA[ITX_MAX_RANK], sB[ITX_MAX_RANK], sC[ITX_MAX_RANK];
+    for (int i = 0; i < rA; ++i) {
+      if (dA[i] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: negative dimension");
+      if (lm.a2b[i] >= 0 && dB[lm.a2b[i]] != dA[i])
+        return set_err(ctx, ITX_ERR_ARG, "contraction: contracted dimensions differ");
+      if (lm.a2c[i] >= 0) dC[lm.a2c[i]] = dA[i];
+    }
+    for (int j = 0; j < rB; ++j)
+      if (lm.b2c[j] >= 0) dC[lm.b2c[j]] = dB[j];
+    if (blkC) {
+      for (int c = 0; c < rC; ++c)
+        if (blkC[size_t(t) * rC + c] != dC[c])
+          return set_err(ctx, ITX_ERR_ARG, "contraction: dimensions of C do not match A and B");
+    }
+    colmajor_strides(rA, dA, sA);
+    colmajor_strides(rB, dB, sB);
+    colmajor_strides(rC, dC, sC);
+    std::vector<Digit> dm, dn, dk;
+    for (int i = 0; i < rA; ++i) {
+      if (lm.a2c[i] >= 0) dm.push_back({dA[i], sA[i], sC[lm.a2c[i]]});
+      else dk.push_back({dA[i], sA[i], sB[lm.a2b[i]]});
+    }
+    for (int j = 0; j < rB; ++j)
+      if (lm.b2c[j] >= 0) dn.push_back({dB[j], sB[j], sC[lm.b2c[j]]});
+    simplify(dm);
+    simplify(dn);
+    simplify(dk);
+    const int64_t M = digits_size(dm), N = digits_size(dn), K = digits_size(dk);
+    if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX)
+      return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: GEMM extent exceeds 2^31");
+    if (M == 0 || N == 0) continue;  // empty C block: nothing to write
+    if (!kc_set) {
+      kc_set = true;
+      akc = !dk.empty() && dk[0].s1 == 1;
+      bkc = !dk.empty() && dk[0].s2 == 1;
+    }
+    GemmTask g{};
+    g.a_off = offs[3 * size_t(t)];
+    g.b_off = offs[3 * size_t(t) + 1];
+    g.c_off = offs[3 * size_t(t) + 2];
+    g.M = int32_t(M);
+    g.N = int32_t(N);
+    g.K = int32_t(K);
+    int64_t t0 = emit_tables(dm, tab);
+    g.tAM = t0;
+    g.tCM = t0 + M;
+    t0 = emit_tables(dn, tab);
+    g.tBN = t0;
+    g.tCN = t0 + N;
+    t0 = emit_tables(dk, tab);
+    g.tAK = t0;
+    g.tBK = t0 + K;
+    flops += 2.0 * double(M) * double(N) * double(K);
+    tasks.push_back(g);
+  }
+
+  // group tasks by C block (stable: keeps the planner's pair order)
+  std::vector<int> idx(tasks.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int x, int y) { return tasks[x].c_off < tasks[y].c_off; });
+  std::vector<GemmTask> st(tasks.size());
+  for (size_t i = 0; i < idx.size(); ++i) st[i] = tasks[idx[i]];
+  struct Grp {
+    int tb, te;
+    int M, N;
+    double ksum;
+  };
+  std::vector<Grp> groups;
+  for (size_t i = 0; i < st.size();) {
+    size_t e = i + 1;
+    while (e < st.size() && st[e].c_off == st[i].c_off) {
+      if (st[e].M != st[i].M || st[e].N != st[i].N)
+        return set_err(ctx, ITX_ERR_ARG, "contraction: tasks writing one C block disagree on its shape");
+      ++e;
+    }
+    double ks = 0;
+    for (size_t q = i; q < e; ++q) ks += st[q].K;
+    groups.push_back({int(i), int(e), st[i].M, st[i].N, ks});
+    i = e;
+  }
+
+  // pick the tile shape: estimated time = max(total/parallel, largest unit)/eff
+  const int cus = std::max(1, ctx->num_cus);
+  const double eff[3] = {1.0, 0.8, 0.55};
+  int best = 0;
+  double best_t = 1e300;
+  for (int ti = 0; ti < 3; ++ti) {
+    const int bt = kTiles[ti];
+    double tot = 0, mx = 0;
+    int64_t nu = 0;
+    for (auto &g : groups) {
+      int64_t tiles = int64_t((g.M + bt - 1) / bt) * ((g.N + bt - 1) / bt);
+      double c = double(bt) * bt * (g.ksum + BK);
+      nu += tiles;
+      tot += c * tiles;
+      mx = std::max(mx, c);
+    }
+    if (nu == 0) continue;
+    double tt = std::max(tot / double(std::min<int64_t>(nu, cus)), mx) / eff[ti];
+    if (tt < best_t * 0.97) {
+      best_t = tt;
+      best = ti;
+    }
+  }
+  const int bt = kTiles[best];
+  std::vector<WorkUnit> units;
+  std::vector<double> ucost;
+  for (auto &g : groups) {
+    for (int m0 = 0; m0 < g.M; m0 += bt)
+      for (int n0 = 0; n0 < g.N; n0 += bt) {
+        units.push_back({g.tb, g.te, m0, n0});
+        ucost.push_back(g.ksum);
+      }
+  }
+  // heaviest units first (the persistent loop then balances like LPT);
+  // stable so equal-cost tiles keep their m-major order
+  {
+    std::vector<int> ui(units.size());
+    std::iota(ui.begin(), ui.end(), 0);
+    std::stable_sort(ui.begin(), ui.end(), [&](int x, int y) { return ucost[x] > ucost[y]; });
+    std::vector<WorkUnit> su(units.size());
+    for (size_t i = 0; i < ui.size(); ++i) su[i] = units[ui[i]];
+    units.swap(su);
+  }
+
+  auto *p = new itx_plan();
+  p->prec = prec;
+  p->tile = best;
+  p->a_kcontig = akc;
+  p->b_kcontig = bkc;
+  p->ntasks = int(st.size());
+  p->nunits = int(units.size());
+  p->flops = flops;
+  const int occ = (bt == 128) ? 2 : 4;
+  p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * occ));
+  if (p->nunits > 0) {
+    size_t bt_bytes = st.size() * sizeof(GemmTask);
+    size_t bu_bytes = units.size() * sizeof(WorkUnit);
+    size_t o_units = (bt_bytes + 255) & ~size_t(255);
+    size_t o_tab = (o_units + bu_bytes + 255) & ~size_t(255);
+    size_t total = o_tab + tab.size() * sizeof(int64_t);
+    std::vector<char> host(total, 0);
+    std::memcpy(host.data(), st.data(), bt_bytes);
+    std::memcpy(host.data() + o_units, units.data(), bu_bytes);
+    if (!tab.empty()) std::memcpy(host.data() + o_tab, tab.data(), tab.size() * sizeof(int64_t));
+    hipError_t e = hipMalloc(&p->dev, total);
+    if (e != hipSuccess) {
+      delete p;
+      return hip_err(ctx, e, "hipMalloc(plan)");
+    }
+    e = hipMemcpyAsync(p->dev, host.data(), total, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      hipFree(p->dev);
+      delete p;
+      return hip_err(ctx, e, "upload(plan)");
+    }
+    p->dev_bytes = total;
+    p->d_tasks = reinterpret_cast<const GemmTask *>(p->dev);
+    p->d_units = reinterpret_cast<const WorkUnit *>(static_cast<char *>(p->dev) + o_units);
+    p->d_tab = reinterpret_cast<const int64_t *>(static_cast<char *>(p->dev) + o_tab);
+  }
+  *out = p;
+  return ITX_OK;
+}
+
+int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *B, double *C, double alpha,
+                double beta) {
+  if (!p) return set_err(ctx, ITX_ERR_ARG, "null plan");
+  if (p->nunits == 0) return ITX_OK;
+  if (p->prec == ITX_PREC_F64)
+    launch_prec<double>(p, ctx->stream, A, B, C, alpha, beta);
+  else
+    launch_prec<float>(p, ctx->stream, A, B, C, alpha, beta);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+namespace {
+
+std::string plan_key(const char *kind, int rA, const int32_t *labA, int rB, const int32_t *labB, int rC,
+                     const int32_t *labC, int ntasks, const int64_t *offs, const int64_t *blkA,
+                     const int64_t *blkB, int prec) {
+  std::string k;
+  k.reserve(64 + 8 * (rA + rB + rC) + size_t(ntasks) * 8 * (3 + rA + rB));
+  auto put = [&](const void *d, size_t n) { k.append(static_cast<const char *>(d), n); };
+  put(kind, std::strlen(kind));
+  put(&rA, sizeof rA);
+  put(&rB, sizeof rB);
+  put(&rC, sizeof rC);
+  put(&prec, sizeof prec);
+  put(&ntasks, sizeof ntasks);
+  put(labA, sizeof(int32_t) * rA);
+  put(labB, sizeof(int32_t) * rB);
+  put(labC, sizeof(int32_t) * rC);
+  put(offs, sizeof(int64_t) * 3 * size_t(ntasks));
+  put(blkA, sizeof(int64_t) * rA * size_t(ntasks));
+  put(blkB, sizeof(int64_t) * rB * size_t(ntasks));
+  return k;
+}
+
+constexpr size_t kPlanCacheMax = 96;
+
+int cached_plan(itx_ctx *ctx, const std::string &key, itx_plan **out,
+                const std::function<int(itx_plan **)> &make) {
+  auto it = ctx->plan_cache.find(key);
+  if (it != ctx->plan_cache.end()) {
+    *out = it->second;
+    return ITX_OK;
+  }
+  itx_plan *p = nullptr;
+  int rc = make(&p);
+  if (rc) return rc;
+  if (ctx->plan_cache.size() >= kPlanCacheMax) {
+    // evict the oldest entry (FIFO); its kernels were enqueued before,
+    // so wait for them before the device tables go away
+    hipStreamSynchronize(ctx->stream);
+    const std::string old = ctx->plan_lru.front();
+    ctx->plan_lru.erase(ctx->plan_lru.begin());
+    auto jt = ctx->plan_cache.find(old);
+    if (jt != ctx->plan_cache.end()) {
+      itx_plan_destroy(ctx, jt->second);
+      ctx->plan_cache.erase(jt);
+    }
+  }
+  ctx->plan_cache[key] = p;
+  ctx->plan_lru.push_back(key);
+  *out = p;
+  return ITX_OK;
+}
+
+}  // namespace
+}  // namespace itx
+
+extern "C" {
+
+int itx_plan_dense(itx_ctx *ctx, int rA, const int64_t *dimsA, const int32_t *labA, int rB,
+                   const int64_t *dimsB, const int32_t *labB, int rC, const int64_t *dimsC,
+                   const int32_t *labC, int prec, itx_plan **out) {
+  if (!ctx || !out) return ITX_ERR_ARG;
+  const int64_t offs[3] = {0, 0, 0};
+  return itx::build_plan(ctx, rA, labA, rB, labB, rC, labC, 1, offs, dimsA, dimsB, dimsC, prec, out);
+}
+
+int itx_plan_qn(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t *labB, int rC,
+                const int32_t *labC, int ntasks, const int64_t *offs, const int64_t *blkdimsA,
+                const int64_t *blkdimsB, int prec, itx_plan **out) {
+  if (!ctx || !out) return ITX_ERR_ARG;
+  return itx::build_plan(ctx, rA, labA, rB, labB, rC, labC, ntasks, offs, blkdimsA, blkdimsB, nullptr, prec,
+                         out);
+}
+
+int itx_plan_execute(itx_ctx *ctx, const itx_plan *plan, const double *A, const double *B, double *C,
+                     double alpha, double beta) {
+  if (!ctx) return ITX_ERR_ARG;
+  return itx::launch_plan(ctx, plan, A, B, C, alpha, beta);
+}
+
+double itx_plan_flops(const itx_plan *plan) { return plan ? plan->flops : 0.0; }
+
+int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan) {
+  if (!plan) return ITX_OK;
+  if (plan->dev) {
+    if (ctx) hipStreamSynchronize(ctx->stream);
+    hipFree(plan->dev);
+  }
+  delete plan;
+  return ITX_OK;
+}
+
+int itx_contract_dense(itx_ctx *ctx, const double *A, int rA, const int64_t *dimsA, const int32_t *labA,
+                       const double *B, int rB, const int64_t *dimsB, const int32_t *labB, double *C, int rC,
+                       const int64_t *dimsC, const int32_t *labC, double alpha, double beta, int prec) {
+  if (!ctx) return ITX_ERR_ARG;
+  const int64_t offs[3] = {0, 0, 0};
+  std::string key = itx::plan_key("D", rA, labA, rB, labB, rC, labC, 1, offs, dimsA, dimsB, prec);
+  key.append(reinterpret_cast<const char *>(dimsC), sizeof(int64_t) * rC);
+  itx_plan *p = nullptr;
+  int rc = itx::cached_plan(ctx, key, &p, [&](itx_plan **o) {
+    return itx::build_plan(ctx, rA, labA, rB, labB, rC, labC, 1, offs, dimsA, dimsB, dimsC, prec, o);
+  });
+  if (rc) return rc;
+  return itx::launch_plan(ctx, p, A, B, C, alpha, beta);
+}
+
+int itx_contract_qn(itx_ctx *ctx, const double *A, int rA, const int32_t *labA, const double *B, int rB,
+                    const int32_t *labB, double *C, int rC, const int32_t *labC, int ntasks,
+                    const int64_t *offs, const int64_t *blkdimsA, const int64_t *blkdimsB, double alpha,
+                    int prec) {
+  if (!ctx) return ITX_ERR_ARG;
+  std::string key = itx::plan_key("Q", rA, labA, rB, labB, rC, labC, ntasks, offs, blkdimsA, blkdimsB, prec);
+  itx_plan *p = nullptr;
+  int rc = itx::cached_plan(ctx, key, &p, [&](itx_plan **o) {
+    return itx::build_plan(ctx, rA, labA, rB, labB, rC, labC, ntasks, offs, blkdimsA, blkdimsB, nullptr, prec,
+                           o);
+  });
+  if (rc) return rc;
+  return itx::launch_plan(ctx, p, A, B, C, alpha, 0.0);
+}
+
+}  // extern "C"

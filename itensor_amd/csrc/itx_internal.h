@@ -1,0 +1,91 @@
+// Internal declarations shared by the libitx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/itx.h"
+
+namespace itx {
+
+// One GEMM of a contraction: C[m,n] (+)= sum_k A[m,k] B[k,n], where the
+// element of A at logical (m,k) lives at a_off + tab[tAM+m] + tab[tAK+k]
+// (and likewise for B and C).  The offset tables carry the index
+// permutations, so no operand is ever permuted in memory.
+struct GemmTask {
+  int64_t a_off, b_off, c_off;
+  int32_t M, N, K, pad0;
+  int64_t tAM, tAK, tBK, tBN, tCM, tCN;
+};
+
+// One output tile of one C block: the tasks [tbeg,tend) all write the
+// same C block and are summed into the tile in registers.
+struct WorkUnit {
+  int32_t tbeg, tend, m0, n0;
+};
+
+}  // namespace itx
+
+struct itx_plan {
+  int prec = ITX_PREC_F64;
+  int tile = 0;        // index into the tile-shape table of the engine
+  bool a_kcontig = false, b_kcontig = false;
+  int ntasks = 0, nunits = 0;
+  int grid = 0;
+  double flops = 0;
+  void *dev = nullptr;  // one device block: tasks | units | tables
+  size_t dev_bytes = 0;
+  const itx::GemmTask *d_tasks = nullptr;
+  const itx::WorkUnit *d_units = nullptr;
+  const int64_t *d_tab = nullptr;
+};
+
+struct itx_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // caching allocator: size -> free blocks; live block -> size
+  std::multimap<size_t, void *> free_blocks;
+  std::unordered_map<void *, size_t> live;
+  size_t cached_bytes = 0;
+  // reduction scratch (partials + results)
+  double *red = nullptr;
+  // plan cache for the one-shot contraction entry points
+  std::unordered_map<std::string, itx_plan *> plan_cache;
+  std::vector<std::string> plan_lru;
+};
+
+namespace itx {
+
+int set_err(itx_ctx *ctx, int code, const std::string &msg);
+int hip_err(itx_ctx *ctx, hipError_t e, const char *where);
+
+// Device allocation through the context's caching pool.
+int pool_alloc(itx_ctx *ctx, size_t bytes, void **p);
+int pool_free(itx_ctx *ctx, void *p);
+
+// Build a contraction plan (dense = one task).  Shared by the dense and
+// QN entry points.  blk*: per-task block dims (ntasks x r).
+int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t *labB,
+               int rC, const int32_t *labC, int ntasks, const int64_t *offs,
+               const int64_t *blkA, const int64_t *blkB, const int64_t *blkC_or_null,
+               int prec, itx_plan **out);
+int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *B, double *C,
+                double alpha, double beta);
+
+constexpr int RED_SLOTS = 2048;  // partial-sum slots in ctx->red
+
+}  // namespace itx
+
+#define ITX_CHECK_HIP(ctx, call)                                   \
+  do {                                                             \
+    hipError_t _e = (call);                                        \
+    if (_e != hipSuccess) return itx::hip_err((ctx), _e, #call);   \
+  } while (0)

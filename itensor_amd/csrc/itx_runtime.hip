@@ -1,0 +1,203 @@
+// Context, stream and caching device allocator of libitx.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "itx_internal.h"
+
+namespace {
+std::mutex g_create_mu;
+std::string g_create_err;  // error text of the last failed itx_create
+}  // namespace
+
+namespace itx {
+
+int set_err(itx_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_err(itx_ctx *ctx, hipError_t e, const char *where) {
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  if (ctx) ctx->err = m;
+  return e == hipErrorOutOfMemory ? ITX_ERR_NOMEM : ITX_ERR_HIP;
+}
+
+int pool_alloc(itx_ctx *ctx, size_t bytes, void **p) {
+  if (bytes == 0) bytes = 8;
+  // round to 256 B so that blocks of nearby sizes are interchangeable
+  bytes = (bytes + 255) & ~size_t(255);
+  auto it = ctx->free_blocks.lower_bound(bytes);
+  if (it != ctx->free_blocks.end() && it->first <= bytes + bytes / 4 + 4096) {
+    *p = it->second;
+    ctx->live[*p] = it->first;
+    ctx->cached_bytes -= it->first;
+    ctx->free_blocks.erase(it);
+    return ITX_OK;
+  }
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    // release the cache and retry once
+    (void)hipGetLastError();
+    hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->free_blocks) hipFree(kv.second);
+    ctx->free_blocks.clear();
+    ctx->cached_bytes = 0;
+    e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return hip_err(ctx, e, "hipMalloc");
+  }
+  ctx->live[q] = bytes;
+  *p = q;
+  return ITX_OK;
+}
+
+int pool_free(itx_ctx *ctx, void *p) {
+  if (!p) return ITX_OK;
+  auto it = ctx->live.find(p);
+  if (it == ctx->live.end()) return set_err(ctx, ITX_ERR_ARG, "itx_free: pointer not from itx_malloc");
+  // Stream order makes reuse safe: every later user of this block is
+  // enqueued on the same stream after the work that used it before.
+  ctx->free_blocks.emplace(it->second, p);
+  ctx->cached_bytes += it->second;
+  ctx->live.erase(it);
+  return ITX_OK;
+}
+
+}  // namespace itx
+
+namespace {
+__global__ void k_fill(double *x, size_t n, double v) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) x[i] = v;
+}
+}  // namespace
+
+extern "C" {
+
+int itx_create(int device, itx_ctx **out) {
+  std::lock_guard<std::mutex> lk(g_create_mu);
+  if (!out) return ITX_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    g_create_err = "itx_create: no HIP device available";
+    return ITX_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_err = "itx_create: device ordinal out of range";
+    return ITX_ERR_ARG;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return ITX_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+    return ITX_ERR_HIP;
+  }
+  auto *ctx = new itx_ctx();
+  ctx->device = device;
+  ctx->num_cus = prop.multiProcessorCount;
+  ctx->stream = nullptr;
+  e = hipMalloc(&ctx->red, sizeof(double) * (itx::RED_SLOTS + 64));
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipMalloc(scratch): ") + hipGetErrorString(e);
+    delete ctx;
+    return ITX_ERR_NOMEM;
+  }
+  *out = ctx;
+  return ITX_OK;
+}
+
+int itx_destroy(itx_ctx *ctx) {
+  if (!ctx) return ITX_OK;
+  hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->plan_cache) itx_plan_destroy(ctx, kv.second);
+  ctx->plan_cache.clear();
+  for (auto &kv : ctx->free_blocks) hipFree(kv.second);
+  for (auto &kv : ctx->live) hipFree(kv.first);
+  hipFree(ctx->red);
+  delete ctx;
+  return ITX_OK;
+}
+
+int itx_set_stream(itx_ctx *ctx, void *stream) {
+  if (!ctx) return ITX_ERR_ARG;
+  ctx->stream = reinterpret_cast<hipStream_t>(stream);
+  return ITX_OK;
+}
+
+void *itx_get_stream(itx_ctx *ctx) { return ctx ? reinterpret_cast<void *>(ctx->stream) : nullptr; }
+
+int itx_synchronize(itx_ctx *ctx) {
+  if (!ctx) return ITX_ERR_ARG;
+  ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ITX_OK;
+}
+
+const char *itx_last_error(itx_ctx *ctx) {
+  if (!ctx) return g_create_err.c_str();
+  return ctx->err.c_str();
+}
+
+int itx_device_cus(itx_ctx *ctx) { return ctx ? ctx->num_cus : 0; }
+
+int itx_malloc(itx_ctx *ctx, size_t bytes, void **ptr) {
+  if (!ctx || !ptr) return ITX_ERR_ARG;
+  return itx::pool_alloc(ctx, bytes, ptr);
+}
+
+int itx_free(itx_ctx *ctx, void *ptr) {
+  if (!ctx) return ITX_ERR_ARG;
+  return itx::pool_free(ctx, ptr);
+}
+
+int itx_trim(itx_ctx *ctx) {
+  if (!ctx) return ITX_ERR_ARG;
+  ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto &kv : ctx->free_blocks) hipFree(kv.second);
+  ctx->free_blocks.clear();
+  ctx->cached_bytes = 0;
+  return ITX_OK;
+}
+
+int itx_memcpy_htod(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (bytes == 0) return ITX_OK;
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ITX_OK;
+}
+
+int itx_memcpy_dtoh(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (bytes == 0) return ITX_OK;
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ITX_OK;
+}
+
+int itx_memcpy_dtod(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (bytes == 0) return ITX_OK;
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return ITX_OK;
+}
+
+int itx_fill(itx_ctx *ctx, double *x, size_t n, double v) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (n == 0) return ITX_OK;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream, x, n, v);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+// FP64 vector kernels of libitx: dot / nrm2 / multi-dot (two-pass,
+// deterministic wavefront reductions), axpy, scal, linear combinations and
+// the permuted add used by ITensor's PlusEQ.
+//
+// Reference call sites: ddot/dnrm2/daxpy/dscal wrappers
+// (tensor/lapack_wrap.cc:26-75,353) reached from norm (itensor.cc:754-766,
+// itdata/dense.cc:150-156), += (itensor.cc:1207-1264, itdata/dense.cc:362-415)
+// and the Davidson dots eltC(dag(V)*q) (iterativesolvers.h:169,315,393).
+#include "itx_internal.h"
+
+namespace itx {
+namespace {
+
+constexpr int VT = 256;
+constexpr int MAXV = 16;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// block-wide sum; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double part[VT / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < VT / 64; ++i) r += part[i];
+  __syncthreads();
+  return r;
+}
+
+struct VPtrs {
+  const double *p[MAXV];
+};
+struct VCoef {
+  double c[MAXV];
+};
+
+// partial[j*nbx + bx] = sum over this block's chunk of V_j . y
+__global__ __launch_bounds__(VT) void k_dot_partial(VPtrs V, const double *__restrict__ y, size_t n,
+                                                    double *__restrict__ partial) {
+  const int j = blockIdx.y;
+  const double *__restrict__ x = V.p[j];
+  double s = 0;
+  const size_t stride = size_t(gridDim.x) * VT;
+  for (size_t i = size_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) s += x[i] * y[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[size_t(j) * gridDim.x + blockIdx.x] = s;
+}
+
+// out[j] = f(sum_b partial[j*nb + b]), f = identity or sqrt
+__global__ __launch_bounds__(VT) void k_dot_final(const double *__restrict__ partial, int nb,
+                                                  double *__restrict__ out, int do_sqrt) {
+  const int j = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += VT) s += partial[size_t(j) * nb + b];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[j] = do_sqrt ? sqrt(s) : s;
+}
+
+__global__ __launch_bounds__(VT) void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y,
+                                             size_t n) {
+  const size_t stride = size_t(gridDim.x) * VT;
+  for (size_t i = size_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(VT) void k_scal(double a, double *__restrict__ x, size_t n) {
+  const size_t stride = size_t(gridDim.x) * VT;
+  for (size_t i = size_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) x[i] *= a;
+}
+
+__global__ __launch_bounds__(VT) void k_lincomb(VPtrs V, VCoef c, int nv, double *__restrict__ y, size_t n) {
+  const size_t stride = size_t(gridDim.x) * VT;
+  for (size_t i = size_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) {
+    double s = 0;
+    for (int j = 0; j < nv; ++j) s += c.c[j] * V.p[j][i];
+    y[i] = s;
+  }
+}
+
+struct PermDesc {
+  int r;
+  int64_t dimY[ITX_MAX_RANK];
+  int64_t sX[ITX_MAX_RANK];  // stride in X of Y's index j
+};
+
+__global__ __launch_bounds__(VT) void k_permute(const double *__restrict__ X, double *__restrict__ Y,
+                                                PermDesc d, size_t n, double alpha, double beta) {
+  const size_t stride = size_t(gridDim.x) * VT;
+  for (size_t i = size_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) {
+    size_t rem = i;
+    int64_t xo = 0;
+    for (int j = 0; j < d.r; ++j) {
+      const int64_t dj = d.dimY[j];
+      const int64_t q = int64_t(rem / size_t(dj));
+      xo += (int64_t(rem) - q * dj) * d.sX[j];
+      rem = size_t(q);
+    }
+    const double v = alpha * X[xo];
+    Y[i] = (beta == 0.0) ? v : v + beta * Y[i];
+  }
+}
+
+unsigned grid_for(size_t n) {
+  size_t b = (n + VT * 4 - 1) / (VT * 4);
+  if (b < 1) b = 1;
+  if (b > 4096) b = 4096;
+  return unsigned(b);
+}
+
+int multi_dot_dev(itx_ctx *ctx, const double *const *V, int nv, const double *y, size_t n, double *dout,
+                  int do_sqrt) {
+  if (nv <= 0) return ITX_OK;
+  if (nv > MAXV) return set_err(ctx, ITX_ERR_UNSUPPORTED, "multi_dot: more than 16 vectors");
+  VPtrs vp{};
+  for (int j = 0; j < nv; ++j) vp.p[j] = V[j];
+  int nbx = int(grid_for(n));
+  const int cap = RED_SLOTS / nv;
+  if (nbx > cap) nbx = cap;
+  hipLaunchKernelGGL(k_dot_partial, dim3(nbx, nv), dim3(VT), 0, ctx->stream, vp, y, n, ctx->red);
+  hipLaunchKernelGGL(k_dot_final, dim3(nv), dim3(VT), 0, ctx->stream, ctx->red, nbx, dout, do_sqrt);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+int to_host(itx_ctx *ctx, const double *dsrc, double *hdst, int count) {
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(hdst, dsrc, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+  ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ITX_OK;
+}
+
+}  // namespace
+}  // namespace itx
+
+extern "C" {
+
+int itx_dot_dev(itx_ctx *ctx, const double *x, const double *y, size_t n, double *dresult) {
+  if (!ctx) return ITX_ERR_ARG;
+  const double *V[1] = {x};
+  return itx::multi_dot_dev(ctx, V, 1, y, n, dresult, 0);
+}
+
+int itx_dot(itx_ctx *ctx, const double *x, const double *y, size_t n, double *result) {
+  if (!ctx || !result) return ITX_ERR_ARG;
+  double *slot = ctx->red + itx::RED_SLOTS;
+  int rc = itx_dot_dev(ctx, x, y, n, slot);
+  if (rc) return rc;
+  return itx::to_host(ctx, slot, result, 1);
+}
+
+int itx_nrm2(itx_ctx *ctx, const double *x, size_t n, double *result) {
+  if (!ctx || !result) return ITX_ERR_ARG;
+  double *slot = ctx->red + itx::RED_SLOTS;
+  const double *V[1] = {x};
+  int rc = itx::multi_dot_dev(ctx, V, 1, x, n, slot, 1);
+  if (rc) return rc;
+  return itx::to_host(ctx, slot, result, 1);
+}
+
+int itx_multi_dot(itx_ctx *ctx, const double *const *V, int nv, const double *y, size_t n, double *out) {
+  if (!ctx || !out || nv < 0 || nv > 16) return ITX_ERR_ARG;
+  if (nv == 0) return ITX_OK;
+  double *slot = ctx->red + itx::RED_SLOTS;
+  int rc = itx::multi_dot_dev(ctx, V, nv, y, n, slot, 0);
+  if (rc) return rc;
+  return itx::to_host(ctx, slot, out, nv);
+}
+
+int itx_axpy(itx_ctx *ctx, double a, const double *x, double *y, size_t n) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (n == 0) return ITX_OK;
+  hipLaunchKernelGGL(itx::k_axpy, dim3(itx::grid_for(n)), dim3(itx::VT), 0, ctx->stream, a, x, y, n);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+int itx_scal(itx_ctx *ctx, double a, double *x, size_t n) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (n == 0) return ITX_OK;
+  hipLaunchKernelGGL(itx::k_scal, dim3(itx::grid_for(n)), dim3(itx::VT), 0, ctx->stream, a, x, n);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+int itx_lincomb(itx_ctx *ctx, const double *const *V, const double *c, int nv, double *y, size_t n) {
+  if (!ctx || nv < 0 || nv > itx::MAXV) return ITX_ERR_ARG;
+  if (n == 0) return ITX_OK;
+  itx::VPtrs vp{};
+  itx::VCoef vc{};
+  for (int j = 0; j < nv; ++j) {
+    vp.p[j] = V[j];
+    vc.c[j] = c[j];
+  }
+  hipLaunchKernelGGL(itx::k_lincomb, dim3(itx::grid_for(n)), dim3(itx::VT), 0, ctx->stream, vp, vc, nv, y, n);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+int itx_permute(itx_ctx *ctx, const double *X, double *Y, int r, const int64_t *dimsY, const int32_t *perm,
+                double alpha, double beta) {
+  if (!ctx || r < 0 || r > ITX_MAX_RANK) return ITX_ERR_ARG;
+  itx::PermDesc d{};
+  d.r = r;
+  // X dims: dimsX[perm[j]] = dimsY[j]; strides of X are column-major in X's order
+  int64_t dimsX[ITX_MAX_RANK];
+  std::vector<int> seen(r, 0);
+  for (int j = 0; j < r; ++j) {
+    if (perm[j] < 0 || perm[j] >= r || seen[perm[j]]) return itx::set_err(ctx, ITX_ERR_ARG, "itx_permute: bad perm");
+    seen[perm[j]] = 1;
+    dimsX[perm[j]] = dimsY[j];
+  }
+  int64_t sX[ITX_MAX_RANK];
+  int64_t s = 1;
+  size_t n = 1;
+  for (int i = 0; i < r; ++i) {
+    sX[i] = s;
+    s *= dimsX[i];
+  }
+  for (int j = 0; j < r; ++j) {
+    d.dimY[j] = dimsY[j];
+    d.sX[j] = sX[perm[j]];
+    n *= size_t(dimsY[j]);
+  }
+  if (n == 0) return ITX_OK;
+  hipLaunchKernelGGL(itx::k_permute, dim3(itx::grid_for(n)), dim3(itx::VT), 0, ctx->stream, X, Y, d, n, alpha,
+                     beta);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+"""ctypes binding of the libitx C ABI (include/itx.h).
+
+This is the Python view of the drop-in boundary: every function here is a
+thin call into libitx.so, the library that the ITensor-compatible C++ host
+layer links against.  Device buffers are passed as raw addresses (for
+example torch tensors' data_ptr()); nothing here computes on the host.
+
+The library must have been built (itensor_amd/build.py); a missing library
+raises ImportError -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libitx.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "itx.h")
+
+ITX_OK = 0
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+
+_lib = None
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+_PROTOS = {
+    "itx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "itx_destroy": (C.c_int, [_vp]),
+    "itx_set_stream": (C.c_int, [_vp, _vp]),
+    "itx_get_stream": (_vp, [_vp]),
+    "itx_synchronize": (C.c_int, [_vp]),
+    "itx_last_error": (C.c_char_p, [_vp]),
+    "itx_device_cus": (C.c_int, [_vp]),
+    "itx_malloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "itx_free": (C.c_int, [_vp, _vp]),
+    "itx_trim": (C.c_int, [_vp]),
+    "itx_memcpy_htod": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "itx_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "itx_memcpy_dtod": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "itx_fill": (C.c_int, [_vp, _vp, C.c_size_t, C.c_double]),
+    "itx_contract_dense": (C.c_int, [_vp, _vp, C.c_int, _i64p, _i32p, _vp, C.c_int, _i64p, _i32p,
+                                     _vp, C.c_int, _i64p, _i32p, C.c_double, C.c_double, C.c_int]),
+    "itx_contract_qn": (C.c_int, [_vp, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p,
+                                  C.c_int, _i64p, _i64p, _i64p, C.c_double, C.c_int]),
+    "itx_plan_dense": (C.c_int, [_vp, C.c_int, _i64p, _i32p, C.c_int, _i64p, _i32p, C.c_int, _i64p,
+                                 _i32p, C.c_int, C.POINTER(_vp)]),
+    "itx_plan_qn": (C.c_int, [_vp, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i64p,
+                              _i64p, _i64p, C.c_int, C.POINTER(_vp)]),
+    "itx_plan_execute": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double]),
+    "itx_plan_flops": (C.c_double, [_vp]),
+    "itx_plan_destroy": (C.c_int, [_vp, _vp]),
+    "itx_dot": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _dp]),
+    "itx_nrm2": (C.c_int, [_vp, _vp, C.c_size_t, _dp]),
+    "itx_dot_dev": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
+    "itx_multi_dot": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, _vp, C.c_size_t, _dp]),
+    "itx_axpy": (C.c_int, [_vp, C.c_double, _vp, _vp, C.c_size_t]),
+    "itx_scal": (C.c_int, [_vp, C.c_double, _vp, C.c_size_t]),
+    "itx_lincomb": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.c_int, _vp, C.c_size_t]),
+    "itx_permute": (C.c_int, [_vp, _vp, _vp, C.c_int, _i64p, _i32p, C.c_double, C.c_double]),
+    "itx_syev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Names of all functions declared in include/itx.h."""
+    with open(HEADER) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(itx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libitx.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _arr(t, vals):
+    vals = list(vals)
+    return (t * max(1, len(vals)))(*vals)
+
+
+class ItxError(RuntimeError):
+    pass
+
+
+class Context:
+    """An itx_ctx bound to one device and one HIP stream."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.itx_create(device, C.byref(h))
+        if rc != ITX_OK:
+            raise ItxError(f"itx_create failed ({rc}): {self.L.itx_last_error(None).decode()}")
+        self.h = h
+        if stream is not None:
+            self.set_stream(stream)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.itx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int, what: str = ""):
+        if rc != ITX_OK:
+            raise ItxError(f"{what} failed ({rc}): {self.L.itx_last_error(self.h).decode()}")
+
+    def set_stream(self, stream: int):
+        self.check(self.L.itx_set_stream(self.h, _vp(stream)), "itx_set_stream")
+
+    def sync(self):
+        self.check(self.L.itx_synchronize(self.h), "itx_synchronize")
+
+    @property
+    def cus(self) -> int:
+        return self.L.itx_device_cus(self.h)
+
+    # ---- contractions (device addresses) ----
+    def contract_dense(self, A: int, dimsA, labA, B: int, dimsB, labB, Cp: int, dimsC, labC,
+                       alpha=1.0, beta=0.0, prec="f64"):
+        rc = self.L.itx_contract_dense(
+            self.h, _vp(A), len(dimsA), _arr(C.c_int64, dimsA), _arr(C.c_int32, labA),
+            _vp(B), len(dimsB), _arr(C.c_int64, dimsB), _arr(C.c_int32, labB),
+            _vp(Cp), len(dimsC), _arr(C.c_int64, dimsC), _arr(C.c_int32, labC),
+            float(alpha), float(beta), PREC[prec])
+        self.check(rc, "itx_contract_dense")
+
+    def contract_qn(self, A: int, labA, B: int, labB, Cp: int, labC, offs, blkA, blkB,
+                    alpha=1.0, prec="f64"):
+        offs = np.ascontiguousarray(offs, dtype=np.int64).ravel()
+        blkA = np.ascontiguousarray(blkA, dtype=np.int64).ravel()
+        blkB = np.ascontiguousarray(blkB, dtype=np.int64).ravel()
+        nt = len(offs) // 3
+        rc = self.L.itx_contract_qn(
+            self.h, _vp(A), len(labA), _arr(C.c_int32, labA), _vp(B), len(labB), _arr(C.c_int32, labB),
+            _vp(Cp), len(labC), _arr(C.c_int32, labC), nt,
+            offs.ctypes.data_as(_i64p), blkA.ctypes.data_as(_i64p), blkB.ctypes.data_as(_i64p),
+            float(alpha), PREC[prec])
+        self.check(rc, "itx_contract_qn")
+
+    def plan_dense(self, dimsA, labA, dimsB, labB, dimsC, labC, prec="f64") -> "Plan":
+        h = _vp()
+        rc = self.L.itx_plan_dense(self.h, len(dimsA), _arr(C.c_int64, dimsA), _arr(C.c_int32, labA),
+                                   len(dimsB), _arr(C.c_int64, dimsB), _arr(C.c_int32, labB),
+                                   len(dimsC), _arr(C.c_int64, dimsC), _arr(C.c_int32, labC),
+                                   PREC[prec], C.byref(h))
+        self.check(rc, "itx_plan_dense")
+        return Plan(self, h)
+
+    def plan_qn(self, labA, labB, labC, offs, blkA, blkB, prec="f64") -> "Plan":
+        offs = np.ascontiguousarray(offs, dtype=np.int64).ravel()
+        blkA = np.ascontiguousarray(blkA, dtype=np.int64).ravel()
+        blkB = np.ascontiguousarray(blkB, dtype=np.int64).ravel()
+        h = _vp()
+        rc = self.L.itx_plan_qn(self.h, len(labA), _arr(C.c_int32, labA), len(labB), _arr(C.c_int32, labB),
+                                len(labC), _arr(C.c_int32, labC), len(offs) // 3,
+                                offs.ctypes.data_as(_i64p), blkA.ctypes.data_as(_i64p),
+                                blkB.ctypes.data_as(_i64p), PREC[prec], C.byref(h))
+        self.check(rc, "itx_plan_qn")
+        return Plan(self, h)
+
+    # ---- vector kernels ----
+    def dot(self, x: int, y: int, n: int) -> float:
+        r = C.c_double()
+        self.check(self.L.itx_dot(self.h, _vp(x), _vp(y), n, C.byref(r)), "itx_dot")
+        return r.value
+
+    def nrm2(self, x: int, n: int) -> float:
+        r = C.c_double()
+        self.check(self.L.itx_nrm2(self.h, _vp(x), n, C.byref(r)), "itx_nrm2")
+        return r.value
+
+    def multi_dot(self, V: list[int], y: int, n: int) -> np.ndarray:
+        out = np.zeros(len(V))
+        ptrs = (_vp * len(V))(*[_vp(v) for v in V])
+        self.check(self.L.itx_multi_dot(self.h, ptrs, len(V), _vp(y), n, out.ctypes.data_as(_dp)),
+                   "itx_multi_dot")
+        return out
+
+    def axpy(self, a: float, x: int, y: int, n: int):
+        self.check(self.L.itx_axpy(self.h, float(a), _vp(x), _vp(y), n), "itx_axpy")
+
+    def scal(self, a: float, x: int, n: int):
+        self.check(self.L.itx_scal(self.h, float(a), _vp(x), n), "itx_scal")
+
+    def lincomb(self, V: list[int], c, y: int, n: int):
+        ptrs = (_vp * len(V))(*[_vp(v) for v in V])
+        cc = np.ascontiguousarray(c, dtype=np.float64)
+        self.check(self.L.itx_lincomb(self.h, ptrs, cc.ctypes.data_as(_dp), len(V), _vp(y), n), "itx_lincomb")
+
+    def permute(self, X: int, Y: int, dimsY, perm, alpha=1.0, beta=0.0):
+        self.check(self.L.itx_permute(self.h, _vp(X), _vp(Y), len(dimsY), _arr(C.c_int64, dimsY),
+                                      _arr(C.c_int32, perm), float(alpha), float(beta)), "itx_permute")
+
+    def syev(self, n: int, A: int, w: int):
+        self.check(self.L.itx_syev(self.h, n, _vp(A), _vp(w)), "itx_syev")
+
+
+class Plan:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    @property
+    def flops(self) -> float:
+        return self.ctx.L.itx_plan_flops(self.h)
+
+    def execute(self, A: int, B: int, Cp: int, alpha=1.0, beta=0.0):
+        self.ctx.check(self.ctx.L.itx_plan_execute(self.ctx.h, self.h, _vp(A), _vp(B), _vp(Cp),
+                                                   float(alpha), float(beta)), "itx_plan_execute")
+
+    def close(self):
+        if self.h:
+            self.ctx.L.itx_plan_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,138 @@
+"""GPU parity of the dense contraction engine (itx_contract_dense / plans)
+against the oracle's contract() (tests call through the C ABI)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle.dense import contract, hpsi_dense, HPSI_STAGES, hpsi_flops, heisenberg_w
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12   # FP64 MFMA vs FP64 BLAS: relative to max |C|
+TOL32 = 2e-5    # FP32 MFMA form
+
+
+def _run(ctx, dev, A, labA, B, labB, labC, prec="f64", alpha=1.0, beta=0.0, C0=None):
+    ref = contract(A, labA, B, labB, labC, alpha=alpha, beta=beta, C=C0)
+    dA, dB = dev.put(A), dev.put(B)
+    dC = dev.put(C0) if C0 is not None else dev.empty(ref.size)
+    ctx.contract_dense(dA.data_ptr(), A.shape, labA, dB.data_ptr(), B.shape, labB,
+                       dC.data_ptr(), ref.shape, labC, alpha, beta, prec)
+    return dev.get(dC, ref.shape), ref
+
+
+# Cases modelled on the reference's contract_test.cc permutation cases
+# (NM1-NM5 non-matrix, M1-M4 matrix-like, rank-0 outputs), with dims chosen
+# so that tile edges are exercised.
+CASES = [
+    # (dimsA, labA, dimsB, labB, labC)
+    ((7, 5), (1, -1), (5, 9), (-1, 2), (1, 2)),               # plain GEMM
+    ((5, 7), (-1, 1), (5, 9), (-1, 2), (1, 2)),               # A^T B
+    ((7, 5), (1, -1), (9, 5), (2, -1), (1, 2)),               # A B^T
+    ((7, 5), (1, -1), (5, 9), (-1, 2), (2, 1)),               # C^T
+    ((3, 4, 5), (1, -1, 2), (4, 6), (-1, 3), (2, 3, 1)),      # NM: mid contracted
+    ((3, 4, 5, 2), (-1, 1, -2, 2), (5, 3, 7), (-2, -1, 3), (3, 1, 2)),
+    ((2, 3, 4, 5), (1, 2, -1, -2), (5, 4, 3), (-2, -1, 3), (1, 2, 3)),
+    ((6, 2, 2, 6), (-1, 2, 3, 4), (6, 5, 6), (-1, 5, 6), (2, 3, 4, 5, 6)),
+    ((33, 17), (-1, -2), (17, 33), (-2, -1), ()),             # full contraction -> scalar
+    ((9,), (1,), (4,), (2,), (1, 2)),                          # outer product
+    ((130, 70), (1, -1), (70, 129), (-1, 2), (1, 2)),          # > one 128 tile
+    ((40, 300), (-1, 1), (40, 257), (-1, 2), (2, 1)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dense_cases(ctx, dev, case, prec):
+    dA, labA, dB, labB, labC = CASES[case]
+    rng = np.random.default_rng(100 + case)
+    A = rng.standard_normal(dA)
+    B = rng.standard_normal(dB)
+    got, ref = _run(ctx, dev, A, labA, B, labB, labC, prec)
+    assert relerr(got, ref) < (TOL64 if prec == "f64" else TOL32), (case, prec)
+
+
+def test_random_permutations(ctx, dev):
+    rng = np.random.default_rng(7)
+    for trial in range(40):
+        rA = int(rng.integers(1, 5))
+        rB = int(rng.integers(1, 5))
+        nc = int(rng.integers(0, min(rA, rB) + 1))
+        dims_c = [int(x) for x in rng.integers(1, 9, size=nc)]
+        da = [int(x) for x in rng.integers(1, 9, size=rA - nc)]
+        db = [int(x) for x in rng.integers(1, 9, size=rB - nc)]
+        la = [-(i + 1) for i in range(nc)] + [10 + i for i in range(rA - nc)]
+        lb = [-(i + 1) for i in range(nc)] + [20 + i for i in range(rB - nc)]
+        dimsA = dims_c + da
+        dimsB = dims_c + db
+        pa = rng.permutation(rA)
+        pb = rng.permutation(rB)
+        dimsA = [dimsA[i] for i in pa]; la = [la[i] for i in pa]
+        dimsB = [dimsB[i] for i in pb]; lb = [lb[i] for i in pb]
+        labC = [l for l in la if l > 0] + [l for l in lb if l > 0]
+        labC = [labC[i] for i in rng.permutation(len(labC))]
+        A = rng.standard_normal(dimsA)
+        B = rng.standard_normal(dimsB)
+        got, ref = _run(ctx, dev, A, la, B, lb, labC)
+        assert relerr(got, ref) < TOL64, (trial, dimsA, la, dimsB, lb, labC)
+
+
+def test_alpha_beta(ctx, dev):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((20, 30))
+    B = rng.standard_normal((30, 25))
+    C0 = rng.standard_normal((20, 25))
+    got, ref = _run(ctx, dev, A, (1, -1), B, (-1, 2), (1, 2), "f64", 0.5, -2.0, C0)
+    assert relerr(got, ref) < TOL64
+
+
+@pytest.mark.parametrize("chi", [16, 64, 200])
+def test_hpsi_chain(ctx, dev, chi):
+    rng = np.random.default_rng(chi)
+    psi = rng.standard_normal((chi, 2, 2, chi))
+    L = rng.standard_normal((chi, 5, chi))
+    R = rng.standard_normal((chi, 5, chi))
+    W = heisenberg_w()
+    ref = hpsi_dense(psi, L, W, W, R)
+    x = dev.put(psi)
+    dims = psi.shape
+    for (la, lb, lc), op in zip(HPSI_STAGES, (L, W, W, R)):
+        # output dims follow from labels
+        outd = [d for d, l in zip(dims, la) if l > 0] + [d for d, l in zip(op.shape, lb) if l > 0]
+        order = [l for l in la if l > 0] + [l for l in lb if l > 0]
+        outd = [outd[order.index(l)] for l in lc]
+        y = dev.empty(int(np.prod(outd)))
+        dop = dev.put(op)
+        ctx.contract_dense(x.data_ptr(), dims, la, dop.data_ptr(), op.shape, lb, y.data_ptr(), outd, lc)
+        x, dims = y, tuple(outd)
+    got = dev.get(x, dims)
+    assert relerr(got, ref) < TOL64
+
+
+def test_plan_reuse_and_flops(ctx, dev):
+    rng = np.random.default_rng(11)
+    la, lb, lc = HPSI_STAGES[0]
+    chi = 48
+    psi = rng.standard_normal((chi, 2, 2, chi))
+    L = rng.standard_normal((chi, 5, chi))
+    ref = contract(psi, la, L, lb, lc)
+    plan = ctx.plan_dense(psi.shape, la, L.shape, lb, ref.shape, lc)
+    assert plan.flops == pytest.approx(2.0 * (4 * chi) * (5 * chi) * chi)
+    dA, dB = dev.put(psi), dev.put(L)
+    for _ in range(3):
+        dC = dev.empty(ref.size)
+        plan.execute(dA.data_ptr(), dB.data_ptr(), dC.data_ptr())
+        assert relerr(dev.get(dC, ref.shape), ref) < TOL64
+    plan.close()
+
+
+def test_bad_labels_raise(ctx, dev):
+    from itensor_amd.itx import ItxError
+
+    A = dev.empty(6, 0.0)
+    with pytest.raises(ItxError):
+        ctx.contract_dense(A.data_ptr(), (2, 3), (1, -1), A.data_ptr(), (2, 3), (-2, 2), A.data_ptr(), (2, 3), (1, 2))
+    with pytest.raises(ItxError):  # contracted dims differ
+        ctx.contract_dense(A.data_ptr(), (2, 3), (1, -1), A.data_ptr(), (2, 3), (-1, 2), A.data_ptr(), (2, 3), (1, 2))
