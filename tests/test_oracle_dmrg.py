@@ -1,0 +1,166 @@
+"""CPU: pin the oracle's DMRG path (truncate, davidson, diag_hermitian,
+denmatDecomp, LocalMPO, sweeps) against the reference's closed forms and
+known answers, exact diagonalisation and reconstruction identities."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle.dmrg import (LocalMPO, SplitMix64, davidson, denmat_decomp, diag_hermitian, dmrg, heisenberg_mpo,
+                         product_mps, random_mps, spin_half_sites, tfim_mpo, truncate)
+from oracle.itensor import QN, Index, contract, dense, norm, qdense
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "heisenberg_ed.json")))
+
+
+def _truncate_p():
+    """The spectrum of decomp_test.cc:59-76 (quickran replaced by our RNG)."""
+    rng = SplitMix64(7)
+    p = np.array([math.exp(-2.0 * n) * (1 + 0.1 * rng.uniform()) for n in range(20)])
+    p[1] = p[2]
+    p[3] = p[5]
+    p[4] = p[5]
+    return p / p.sum()
+
+
+def test_truncate_reference_cases():
+    """decomp_test.cc:78-137, cases 0-4."""
+    p = _truncate_p()
+    assert truncate(p, 40, 1, 0.0)[0].size == 20            # case 0
+    assert truncate(p, 10, 1, 0.0)[0].size == 10            # case 1
+    assert truncate(p, 8, 1, 0.0)[0].size == 8              # case 2
+    assert truncate(p, 40, 10, 0.01)[0].size == 10          # case 3
+    P, te, *_ = truncate(p, 40, 1, 1e-5)                    # case 4
+    m = P.size
+    assert abs(te - p[m:].sum()) < 1e-5 and te < 1e-5
+
+
+def test_truncate_respects_degeneracy():
+    p = np.array([0.5, 0.2, 0.1, 0.1, 0.1])
+    # maxdim falls inside the degenerate triple: drop it entirely
+    assert truncate(p, 3, 1, 0.0, respect_degenerate=True)[0].size == 2
+    # cutoff lands inside it but maxdim allows it: keep all three
+    assert truncate(p, 10, 1, 0.25, respect_degenerate=True)[0].size == 5
+    assert truncate(p, 10, 1, 0.25, respect_degenerate=False)[0].size == 3
+
+
+def _neel_bond_problem(qn):
+    """iterativesolvers_test.cc:43-65: N=4 Heisenberg, Neel product state,
+    Davidson on bond (2,3) with MaxIter=9 -> -(1/4 + 1/sqrt 2)."""
+    N = 4
+    sites = spin_half_sites(N, qn)
+    H = heisenberg_mpo(sites)
+    psi = product_mps(sites, [0 if (i % 2 == 1) else 1 for i in range(1, N + 1)])
+    PH = LocalMPO(H)
+    PH.position(2, psi)
+    phi = psi(2) * psi(3)
+    return davidson(PH.product, phi, {"MaxIter": 9})[0]
+
+
+@pytest.mark.parametrize("qn", [False, True])
+def test_davidson_closed_form(qn):
+    assert abs(_neel_bond_problem(qn) - (-(0.25 + 1 / math.sqrt(2)))) < 1e-10
+
+
+def test_davidson_custom_linear_map():
+    """iterativesolvers_test.cc:168-184: symmetric map, residual ~ 0."""
+    rng = np.random.default_rng(3)
+    a1, a2, a3 = Index(3, "a1"), Index(4, "a2"), Index(5, "a3")
+    M = rng.standard_normal((60, 60))
+    M = 0.5 * (M + M.T)
+    A = dense([a1.prime(), a2.prime(), a3.prime(), a1, a2, a3], M.reshape((3, 4, 5, 3, 4, 5), order="F"))
+    x = dense([a1, a2, a3], rng.standard_normal((3, 4, 5)))
+    lam, v, _ = davidson(lambda t: (A * t).noprime(), x, {"MaxIter": 40, "ErrGoal": 1e-14})
+    r = (A * v).noprime() - v * lam
+    assert norm(r) / norm(v) < 1e-6
+    assert abs(lam - np.linalg.eigvalsh(M)[0]) < 1e-9
+
+
+def test_diag_hermitian_reconstruction_dense_and_qn():
+    rng = np.random.default_rng(4)
+    i = Index(12, "i")
+    X = rng.standard_normal((12, 12))
+    rho = dense([i, i.prime()], X @ X.T)
+    U, D, spec = diag_hermitian(rho, {})
+    R = U.d @ np.diag(D) @ U.d.T
+    assert np.max(np.abs(R - rho.d)) < 1e-12 * np.max(np.abs(rho.d))
+    assert np.all(np.diff(D) <= 0)
+    # QN block-diagonal density matrix
+    j = Index([(QN(0), 3), (QN(1), 4), (QN(-1), 2)], "j")
+    a = qdense([j, Index([(QN(0), 2), (QN(-1), 3), (QN(1), 2)], "k").dag()], QN(0), rng=rng)
+    rho = a * a.dag().prime(1, j)
+    U, D, spec = diag_hermitian(rho, {})
+    Ud = U.to_dense_array()
+    # U^T rho U is diagonal with the kept eigenvalues (in block order)
+    rd = rho.to_dense_array()
+    proj = Ud.T @ rd @ Ud
+    assert np.allclose(proj, np.diag(np.diag(proj)), atol=1e-12)
+    assert np.allclose(sorted(np.diag(proj), reverse=True), D, atol=1e-12)
+
+
+def test_qn_denmat_decomp_reconstructs():
+    """decomp_test.cc:585-625: QN denmatDecomp reconstructs AA to 1e-11 with
+    non-negative kept eigenvalues."""
+    rng = np.random.default_rng(5)
+    S1 = Index([(QN(1), 1), (QN(-1), 1)], "S1")
+    S2 = Index([(QN(1), 1), (QN(-1), 1)], "S2")
+    L1 = Index([(QN(2), 3), (QN(1), 4), (QN(0), 8), (QN(-1), 4), (QN(-2), 2)], "L1")
+    L2 = Index([(QN(2), 4), (QN(1), 6), (QN(0), 10), (QN(-1), 4), (QN(-2), 3)], "L2")
+    L3 = Index([(QN(2), 2), (QN(0), 4), (QN(-2), 2)], "L3")
+    A1 = qdense([L1, S1, L2], QN(), rng=rng)
+    A2 = qdense([L2.dag(), S2, L3], QN(), rng=rng)
+    AA = A1 * A2
+    AA = AA * (-1.0 / norm(AA))
+    nA1, nA2, spec = denmat_decomp(AA, A1, A2, True, {"Truncate": True})
+    AAres = nA1 * nA2
+    diff = AA - AAres
+    assert norm(diff) < 1e-11
+    assert np.all(spec["eigs"] >= 0)
+
+
+def test_four_site_dmrg_energy():
+    """Exact 4-site energy -1.6160254038 (iterativesolvers_test.cc:45)."""
+    sites = spin_half_sites(4, False)
+    H = heisenberg_mpo(sites)
+    psi0 = random_mps(sites, 4, seed=1)
+    sw = [{"MaxDim": 10, "Cutoff": 1e-12}] * 4
+    E, _ = dmrg(H, psi0, sw)
+    assert abs(E - (-1.6160254038)) < 1e-9
+    assert abs(E - GOLD["energies"]["4"]) < 1e-10
+
+
+def test_four_site_qn_dmrg_energy():
+    sites = spin_half_sites(4, True)
+    H = heisenberg_mpo(sites)
+    psi0 = product_mps(sites, [0, 1, 0, 1])
+    E, _ = dmrg(H, psi0, [{"MaxDim": 10, "Cutoff": 1e-12}] * 4)
+    assert abs(E - GOLD["energies"]["4"]) < 1e-10
+
+
+def test_heisenberg_n20_config1_vs_ed():
+    """Config 1: N=20, random MPS chi=10 (seed), sweeps maxdim
+    [10,20,100,100,200], cutoff 1e-10, niter 2; vs exact diagonalisation."""
+    sites = spin_half_sites(20, False)
+    H = heisenberg_mpo(sites)
+    psi0 = random_mps(sites, 10, seed=20261202 + 1)
+    sw = [{"MaxDim": m, "Cutoff": 1e-10, "Niter": 2} for m in (10, 20, 100, 100, 200)]
+    trace = []
+    E, psi = dmrg(H, psi0, sw, record=trace.append)
+    assert len(trace) == 5 * 2 * 19
+    assert abs(E - GOLD["energies"]["20"]) < 1e-8
+    assert all(t["nprod"] == 3 for t in trace if t["sweep"] > 1)
+
+
+def test_tfim_critical_n32_closed_form():
+    """mpo_test.cc:541-570: critical TFIM, N=32, sweeps maxdim 10,20,30,
+    cutoff 1e-12; exact E = (1 - 1/sin(pi/(2(2N+1))))/4."""
+    N = 32
+    sites = spin_half_sites(N, False)
+    H = tfim_mpo(sites, 0.5)
+    psi0 = product_mps(sites, [0] * N)
+    sw = [{"MaxDim": m, "Cutoff": 1e-12} for m in (10, 20, 30, 30, 30)]
+    E, _ = dmrg(H, psi0, sw)
+    exact = (1.0 - 1.0 / math.sin(math.pi / (2 * (2 * N + 1)))) / 4
+    assert abs((E - exact) / exact) < 1e-5  # CHECK_CLOSE: |x| < 1e-5
