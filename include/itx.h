@@ -161,12 +161,31 @@ int itx_lincomb(itx_ctx *ctx, const double *const *V, const double *c, int nv,
 int itx_permute(itx_ctx *ctx, const double *X, double *Y, int r,
                 const int64_t *dimsY, const int32_t *perm, double alpha, double beta);
 
+/* ---- batched strided block copy ----------------------------------------
+ * For each of ntasks tasks t, with desc_t = desc + t*(2+3r) holding
+ * [src_off, dst_off, dims[r], src_strides[r], dst_strides[r]]:
+ *   Y[dst_off + sum_j i_j*dst_strides[j]] =
+ *       alpha * X[src_off + sum_j i_j*src_strides[j]] + beta * Y[...]
+ * over all multi-indices i < dims (beta == 0 never reads Y).  One launch
+ * moves every block of a QDense PlusEQ (itdata/qdense.cc:506-550, incl. the
+ * block-set widening of :582-663) or of a QCombiner combine / uncombine
+ * (itdata/qcombiner.cc:125-300).  Tasks must not write overlapping
+ * elements. */
+int itx_block_copy(itx_ctx *ctx, const double *X, double *Y, int ntasks, int r,
+                   const int64_t *desc, double alpha, double beta);
+
 /* ---- symmetric eigensolver (FP64) --------------------------------------
  * Eigen-decomposition of the symmetric n x n column-major matrix A (device):
  * on return w holds the eigenvalues in DESCENDING order and A the matching
  * orthonormal eigenvectors as columns.  This is what diagHermitian obtains
  * from dsyev on -M (tensor/algs_impl.h:122-135, tensor/algs.cc:34-41). */
 int itx_syev(itx_ctx *ctx, int n, double *A, double *w);
+/* nmat independent problems in one launch: matrix b (n[b] x n[b]) at
+ * A + a_offs[b], eigenvalues to w + w_offs[b].  The per-block
+ * diagonalisation of a block-diagonal QN density matrix
+ * (hermitian.cc:206-257). */
+int itx_syev_batched(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
+                     const int64_t *w_offs, double *w);
 
 #ifdef __cplusplus
 }

@@ -131,15 +131,20 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
   }
 
   // ---- 3. implicit QL on (d, e), rotations applied to Z's columns ----
+  // Deflation test of tql2: |e_m| small against the largest |d|+|e| seen so
+  // far (tst1), not only against its neighbours, so eigenvalues near zero
+  // (rank-deficient density matrices) converge to absolute accuracy.
   int failed = 0;
+  double tst1 = 0;  // used by thread 0 only
   for (int l = 0; l < n && !failed; ++l) {
     int iter = 0;
+    if (tid == 0) tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
     while (true) {
       if (tid == 0) {
         int m = l;
         for (; m < n - 1; ++m) {
           const double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+          if (fabs(e[m]) <= DBL_EPSILON * fmax(dd, tst1)) break;
         }
         sh_m = m;
         sh_nrot = 0;
@@ -275,4 +280,11 @@ extern "C" int itx_syev(itx_ctx *ctx, int n, double *A, double *w) {
   const int32_t ns[1] = {n};
   const int64_t ao[1] = {0}, wo[1] = {0};
   return itx::syev_batched(ctx, 1, ns, ao, A, wo, w);
+}
+
+extern "C" int itx_syev_batched(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
+                                const int64_t *w_offs, double *w) {
+  if (!ctx || nmat < 0) return ITX_ERR_ARG;
+  if (nmat == 0) return ITX_OK;
+  return itx::syev_batched(ctx, nmat, n, a_offs, A, w_offs, w);
 }

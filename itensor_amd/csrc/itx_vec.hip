@@ -6,6 +6,8 @@
 // (tensor/lapack_wrap.cc:26-75,353) reached from norm (itensor.cc:754-766,
 // itdata/dense.cc:150-156), += (itensor.cc:1207-1264, itdata/dense.cc:362-415)
 // and the Davidson dots eltC(dag(V)*q) (iterativesolvers.h:169,315,393).
+#include <cstring>
+
 #include "itx_internal.h"
 
 namespace itx {
@@ -104,6 +106,36 @@ __global__ __launch_bounds__(VT) void k_permute(const double *__restrict__ X, do
     const double v = alpha * X[xo];
     Y[i] = (beta == 0.0) ? v : v + beta * Y[i];
   }
+}
+
+// one task of itx_block_copy, decoded from the int64 descriptor array
+__global__ __launch_bounds__(VT) void k_block_copy(const double *__restrict__ X, double *__restrict__ Y,
+                                                   const int64_t *__restrict__ desc, const int64_t *__restrict__ start,
+                                                   int ntasks, int r, double alpha, double beta) {
+  // blockIdx.x walks a flattened (task, chunk) space: start[t] = first chunk of t
+  const int64_t chunk = blockIdx.x;
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {  // last t with start[t] <= chunk
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= chunk) lo = mid; else hi = mid - 1;
+  }
+  const int t = lo;
+  const int64_t *d = desc + int64_t(t) * (2 + 3 * r);
+  const int64_t *dims = d + 2, *ss = d + 2 + r, *ds = d + 2 + 2 * r;
+  int64_t n = 1;
+  for (int j = 0; j < r; ++j) n *= dims[j];
+  const int64_t i = (chunk - start[t]) * VT + threadIdx.x;
+  if (i >= n) return;
+  int64_t rem = i, so = d[0], dof = d[1];
+  for (int j = 0; j < r; ++j) {
+    const int64_t q = rem / dims[j];
+    const int64_t x = rem - q * dims[j];
+    so += x * ss[j];
+    dof += x * ds[j];
+    rem = q;
+  }
+  const double v = alpha * X[so];
+  Y[dof] = (beta == 0.0) ? v : v + beta * Y[dof];
 }
 
 unsigned grid_for(size_t n) {
@@ -234,3 +266,44 @@ int itx_permute(itx_ctx *ctx, const double *X, double *Y, int r, const int64_t *
 }
 
 }  // extern "C"
+
+extern "C" int itx_block_copy(itx_ctx *ctx, const double *X, double *Y, int ntasks, int r, const int64_t *desc,
+                              double alpha, double beta) {
+  if (!ctx || ntasks < 0 || r < 0 || r > ITX_MAX_RANK) return ITX_ERR_ARG;
+  if (ntasks == 0) return ITX_OK;
+  const size_t per = size_t(2 + 3 * r);
+  std::vector<int64_t> start(ntasks);
+  int64_t chunks = 0;
+  for (int t = 0; t < ntasks; ++t) {
+    int64_t n = 1;
+    for (int j = 0; j < r; ++j) {
+      const int64_t dj = desc[t * per + 2 + j];
+      if (dj < 0) return itx::set_err(ctx, ITX_ERR_ARG, "itx_block_copy: negative dim");
+      n *= dj;
+    }
+    start[t] = chunks;
+    chunks += (n + itx::VT - 1) / itx::VT;
+  }
+  if (chunks == 0) return ITX_OK;
+  if (chunks > 0x7fffffff) return itx::set_err(ctx, ITX_ERR_UNSUPPORTED, "itx_block_copy: too large");
+  void *buf = nullptr;
+  const size_t dbytes = sizeof(int64_t) * per * ntasks;
+  const size_t sbytes = sizeof(int64_t) * ntasks;
+  int rc = itx::pool_alloc(ctx, dbytes + sbytes, &buf);
+  if (rc) return rc;
+  std::vector<int64_t> host(per * ntasks + ntasks);
+  std::memcpy(host.data(), desc, dbytes);
+  std::memcpy(host.data() + per * ntasks, start.data(), sbytes);
+  hipError_t e = hipMemcpyAsync(buf, host.data(), dbytes + sbytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    const int64_t *dd = static_cast<const int64_t *>(buf);
+    hipLaunchKernelGGL(itx::k_block_copy, dim3(unsigned(chunks)), dim3(itx::VT), 0, ctx->stream, X, Y, dd,
+                       dd + per * ntasks, ntasks, r, alpha, beta);
+    e = hipGetLastError();
+  }
+  // the staging vector must outlive the (possibly asynchronous) upload
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  itx::pool_free(ctx, buf);
+  if (e != hipSuccess) return itx::hip_err(ctx, e, "itx_block_copy");
+  return ITX_OK;
+}

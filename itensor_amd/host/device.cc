@@ -1,0 +1,64 @@
+// Process-wide libitx context of the host layer, device buffers and the
+// contraction-precision setting.
+#include <mutex>
+
+#include "itensor.h"
+
+namespace itensor {
+
+namespace {
+std::mutex g_mu;
+itx_ctx *g_ctx = nullptr;
+int g_device = 0;
+int g_prec = ITX_PREC_F64;
+}  // namespace
+
+void check(int rc, const char *what) {
+  if (rc != ITX_OK) {
+    const char *m = itx_last_error(g_ctx);
+    throw ITError(std::string(what) + " failed (" + std::to_string(rc) + "): " + (m ? m : ""));
+  }
+}
+
+itx_ctx *ctx() {
+  if (g_ctx) return g_ctx;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx) {
+    itx_ctx *c = nullptr;
+    int rc = itx_create(g_device, &c);
+    if (rc != ITX_OK) throw ITError(std::string("itx_create failed: ") + itx_last_error(nullptr));
+    g_ctx = c;
+  }
+  return g_ctx;
+}
+
+void setDevice(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_ctx) throw ITError("setDevice: context already created");
+  g_device = device;
+}
+
+void setStream(void *stream) { check(itx_set_stream(ctx(), stream), "itx_set_stream"); }
+
+int contractPrecision() { return g_prec; }
+void setContractPrecision(int prec) { g_prec = prec; }
+
+DevBuf::DevBuf(size_t n_) : n(n_) {
+  void *q = nullptr;
+  check(itx_malloc(ctx(), sizeof(double) * (n_ ? n_ : 1), &q), "itx_malloc");
+  p = static_cast<double *>(q);
+}
+
+DevBuf::~DevBuf() {
+  if (p && g_ctx) itx_free(g_ctx, p);
+}
+
+DevPtr devAlloc(size_t n) { return std::make_shared<DevBuf>(n); }
+
+DevPtr devZeros(size_t n) {
+  auto d = devAlloc(n);
+  if (n) check(itx_fill(ctx(), d->p, n, 0.0), "itx_fill");
+  return d;
+}
+
+}  // namespace itensor

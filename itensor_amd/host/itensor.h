@@ -1,0 +1,308 @@
+// ITensor-compatible host layer of the MI355X DMRG engine.
+//
+// The public names and semantics follow ITensor v3's C++ API for the DMRG
+// hot path (Index / QN / ITensor operator* and +=, norm, combiner,
+// diag_hermitian, denmatDecomp, davidson, LocalOp / LocalMPO, Sweeps,
+// dmrg), so that driver code written against the reference compiles
+// against this header with the same meaning.  Tensor data lives in device
+// memory; every arithmetic step is a call into libitx through the C ABI of
+// include/itx.h.  Host code only does the integer bookkeeping the reference
+// does on the CPU (labels, QN block lists, truncation decisions).
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <variant>
+#include <vector>
+
+#include "itx.h"
+
+namespace itensor {
+
+using Real = double;
+
+enum Arrow { In = -1, Out = 1 };
+enum Direction { Fromleft, Fromright, NoDir };
+
+// Errors of the library are thrown (the reference aborts in Error(),
+// util/error.h:26,54-75; a library must not abort its host process).
+class ITError : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+
+// ---------------------------------------------------------------- Args --
+// Named-argument bag (util/args.h:50-160), with the keys of the DMRG path.
+class Args {
+ public:
+  using Val = std::variant<bool, long, double, std::string>;
+  Args() = default;
+  Args(std::initializer_list<std::pair<const std::string, Val>> il) : m_(il) {}
+  bool defined(std::string const &k) const { return m_.count(k) > 0; }
+  Args &add(std::string const &k, Val v) {
+    m_[k] = std::move(v);
+    return *this;
+  }
+  long getInt(std::string const &k, long def) const;
+  long getInt(std::string const &k) const;
+  double getReal(std::string const &k, double def) const;
+  double getReal(std::string const &k) const;
+  bool getBool(std::string const &k, bool def) const;
+  std::string getString(std::string const &k, std::string const &def) const;
+
+ private:
+  std::map<std::string, Val> m_;
+};
+
+// ------------------------------------------------------------------ QN --
+// Up to four named U(1) charges (qn.h:37-180); missing names count as 0.
+class QN {
+ public:
+  QN() = default;
+  explicit QN(int v) { set("", v); }
+  QN(std::string const &name, int v) { set(name, v); }
+  QN(std::string const &n1, int v1, std::string const &n2, int v2) {
+    set(n1, v1);
+    add(n2, v2);
+  }
+  int val(std::string const &name) const;
+  QN operator+(QN const &o) const;
+  QN operator-(QN const &o) const { return *this + (-o); }
+  QN operator-() const;
+  bool operator==(QN const &o) const { return q_ == o.q_; }
+  bool operator!=(QN const &o) const { return !(*this == o); }
+  std::vector<std::pair<std::string, int>> const &store() const { return q_; }
+  std::string str() const;
+
+ private:
+  void set(std::string const &n, int v);
+  void add(std::string const &n, int v);
+  std::vector<std::pair<std::string, int>> q_;  // sorted by name, no zeros
+};
+inline QN operator*(QN const &q, Arrow a) { return a == Out ? q : -q; }
+
+// --------------------------------------------------------------- Index --
+using QNBlocks = std::vector<std::pair<QN, long>>;
+
+// Index{id, dim, dir, tags, prime level, QN sectors} (index.h:78-282).
+// Equality compares id, tags and prime level (index.cc:154-157).
+class Index {
+ public:
+  Index() = default;
+  explicit Index(long dim, std::string const &tags = "");
+  Index(QNBlocks qns, Arrow dir, std::string const &tags = "");
+  // full description (used by bindings and tests)
+  Index(uint64_t id, long dim, Arrow dir, int plev, std::string const &tags, QNBlocks qns);
+
+  uint64_t id() const { return id_; }
+  long dim() const { return dim_; }
+  Arrow dir() const { return dir_; }
+  int primeLevel() const { return plev_; }
+  std::string const &tags() const { return tags_; }
+  bool hasTag(std::string const &t) const;
+  bool hasQNs() const { return !qns_.empty(); }
+  long nblock() const { return hasQNs() ? long(qns_.size()) : 1; }
+  long blocksize0(long b) const { return hasQNs() ? qns_[b].second : dim_; }
+  QN const &qn0(long b) const { return qns_.at(b).first; }  // 0-based sector
+  long blockStart(long b) const;
+  QNBlocks const &qnBlocks() const { return qns_; }
+  explicit operator bool() const { return id_ != 0; }
+
+  Index &prime(int inc = 1) {
+    plev_ += inc;
+    return *this;
+  }
+  Index &setPrime(int p) {
+    plev_ = p;
+    return *this;
+  }
+  Index &noPrime() { return setPrime(0); }
+  Index &dag() {
+    dir_ = Arrow(-int(dir_));
+    return *this;
+  }
+  Index &setTags(std::string const &t);
+  Index &removeQNs() {
+    qns_.clear();
+    return *this;
+  }
+  std::string str() const;
+
+ private:
+  uint64_t id_ = 0;
+  long dim_ = 0;
+  Arrow dir_ = Out;
+  int plev_ = 0;
+  std::string tags_;  // canonical: sorted, comma separated
+  QNBlocks qns_;
+};
+bool operator==(Index const &a, Index const &b);
+inline bool operator!=(Index const &a, Index const &b) { return !(a == b); }
+inline Index prime(Index i, int inc = 1) { return i.prime(inc); }
+inline Index dag(Index i) { return i.dag(); }
+inline Index noPrime(Index i) { return i.noPrime(); }
+inline long dim(Index const &i) { return i.dim(); }
+uint64_t newIndexId();
+
+using IndexSet = std::vector<Index>;
+long dim(IndexSet const &is);
+int indexPosition(IndexSet const &is, Index const &I);
+bool hasQNs(IndexSet const &is);
+
+using Block = std::vector<int>;
+// reverse-lexicographic block order (itdata/qdense.cc:60-65)
+bool blockLess(Block const &a, Block const &b);
+struct BlOf {
+  Block block;
+  int64_t offset;
+};
+using BlockOffsets = std::vector<BlOf>;
+QN calcDiv(IndexSet const &is, Block const &b);
+// getBlockOffsets (itdata/qdense.cc:133-173)
+std::pair<BlockOffsets, int64_t> getBlockOffsets(IndexSet const &is, QN const &div);
+
+// ------------------------------------------------------------- device --
+// The process-wide libitx context used by the host layer (device 0, the
+// null stream unless setStream is called).
+itx_ctx *ctx();
+void setStream(void *stream);
+void setDevice(int device);
+void check(int rc, const char *what);
+// contraction precision of the H*psi products (ITX_PREC_*); everything
+// else on the path always runs in FP64
+int contractPrecision();
+void setContractPrecision(int prec);
+struct PrecisionScope {
+  int saved;
+  explicit PrecisionScope(int p) : saved(contractPrecision()) { setContractPrecision(p); }
+  ~PrecisionScope() { setContractPrecision(saved); }
+};
+
+struct DevBuf {
+  double *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n_);
+  ~DevBuf();
+  DevBuf(DevBuf const &) = delete;
+  DevBuf &operator=(DevBuf const &) = delete;
+};
+using DevPtr = std::shared_ptr<DevBuf>;
+DevPtr devAlloc(size_t n);
+DevPtr devZeros(size_t n);
+
+// ------------------------------------------------------------ storage --
+enum class SType { None, Dense, QDense, Combiner, QCombiner };
+
+struct QCombInfo {
+  IndexSet inds;  // the combined indices (original arrows)
+  // sector numbers of the combined indices -> (combined sector, start, end)
+  std::map<Block, std::tuple<int, long, long>> ranges;
+  int nsectors = 0;
+};
+
+struct Storage {
+  SType type = SType::None;
+  DevPtr data;  // Dense / QDense values
+  size_t size = 0;
+  BlockOffsets offsets;  // QDense, sorted in block order
+  std::shared_ptr<QCombInfo> qc;
+};
+
+// -------------------------------------------------------------- ITensor --
+class ITensor {
+ public:
+  ITensor() = default;
+  ITensor(IndexSet is, std::shared_ptr<Storage> st) : is_(std::move(is)), store_(std::move(st)) {}
+
+  explicit operator bool() const { return store_ && store_->type != SType::None; }
+  IndexSet const &inds() const { return is_; }
+  int order() const { return int(is_.size()); }
+  Storage const &store() const { return *store_; }
+  std::shared_ptr<Storage> const &storePtr() const { return store_; }
+  bool isQN() const { return store_ && store_->type == SType::QDense; }
+  bool isCombiner() const {
+    return store_ && (store_->type == SType::Combiner || store_->type == SType::QCombiner);
+  }
+
+  // contraction (itensor.cc:935-981)
+  ITensor &operator*=(ITensor const &R);
+  ITensor &operator*=(Real r);
+  ITensor &operator/=(Real r) { return operator*=(1.0 / r); }
+  // daxpy / += / -= (itensor.cc:1207-1264)
+  ITensor &operator+=(ITensor const &R);
+  ITensor &operator-=(ITensor const &R);
+
+  // index manipulation
+  ITensor &noPrime();
+  ITensor &prime(int inc = 1);
+  ITensor &prime(Index const &I, int inc = 1);
+  ITensor &dag();
+  ITensor &setTags(std::string const &tags, Index const &I);
+  ITensor &replaceInd(Index const &from, Index const &to);
+
+  // order-0 value (eltC of a scalar ITensor)
+  Real elt() const;
+  // make storage exclusive (copy-on-write, itdata/itdata.h:126-328)
+  void detach();
+
+  size_t nnz() const;
+  long nnzblocks() const;
+
+ private:
+  IndexSet is_;
+  std::shared_ptr<Storage> store_;
+};
+
+ITensor operator*(ITensor L, ITensor const &R);
+ITensor operator*(ITensor T, Real r);
+ITensor operator*(Real r, ITensor T);
+ITensor operator+(ITensor L, ITensor const &R);
+ITensor operator-(ITensor L, ITensor const &R);
+void daxpy(ITensor &L, ITensor const &R, Real alpha);
+
+Real norm(ITensor const &T);
+// eltC(dag(A)*B) for real tensors
+Real dotDag(ITensor const &A, ITensor const &B);
+ITensor dag(ITensor T);
+ITensor prime(ITensor T, int inc = 1);
+ITensor prime(ITensor T, Index const &I, int inc = 1);
+ITensor noPrime(ITensor T);
+ITensor removeQNs(ITensor const &T);
+Index commonIndex(ITensor const &A, ITensor const &B);
+bool hasIndex(ITensor const &T, Index const &I);
+QN div(ITensor const &T);
+
+// combiner (itensor.cc:1468-1560)
+std::tuple<ITensor, Index> combiner(IndexSet const &inds, Args const &args = Args());
+
+// construction / transfer
+ITensor denseFromHost(IndexSet const &is, std::vector<double> const &colmajor);
+// QDense with the given blocks (sorted) and flat block store
+ITensor qdenseFromHost(IndexSet const &is, BlockOffsets const &offs, std::vector<double> const &store);
+ITensor qdenseZeros(IndexSet const &is, QN const &div);
+std::vector<double> storageToHost(ITensor const &T);
+void randomize(ITensor &T, uint64_t seed);
+
+// ------------------------------------------------------------- decomps --
+struct Spectrum {
+  std::vector<Real> eigs;
+  Real truncerr = 0;
+  Real truncerr_() const { return truncerr; }
+};
+
+// truncate (decomp.cc:305-463): P descending, resized in place
+std::tuple<Real, Real, Real, int> truncate(std::vector<Real> &P, long maxdim, long mindim, Real cutoff,
+                                           bool absoluteCutoff, bool doRelCutoff, Args const &args);
+// diag_hermitian (hermitian.cc:40-440)
+Spectrum diag_hermitian(ITensor rho, ITensor &U, ITensor &D, Args const &args);
+// denmatDecomp without noise (decomp.h:316-418)
+Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args);
+
+}  // namespace itensor

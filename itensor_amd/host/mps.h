@@ -1,0 +1,128 @@
+// MPS / MPO containers, the projected operator (LocalOp, LocalMPO), Sweeps
+// and the two-site DMRG driver of the host layer.
+//
+// Reference behaviour: mps/localop.h:324-365 (product), mps/localmpo.h
+// (position :443-470, makeL/makeR :542-618), mps/mps_impl.h:21-99
+// (svdBond, density-matrix route), mps/sweeps.h (Sweeps :42-145,
+// sweepnext :489-500), mps/dmrg.h:336-467 (DMRGWorker).
+#pragma once
+
+#include <chrono>
+#include <functional>
+
+#include "iterativesolvers.h"
+#include "itensor.h"
+
+namespace itensor {
+
+class MPS {
+ public:
+  MPS() = default;
+  explicit MPS(std::vector<ITensor> A, int leftLim = 0, int rightLim = 2)
+      : A_(std::move(A)), llim_(leftLim), rlim_(rightLim) {}
+  int length() const { return int(A_.size()); }
+  ITensor const &operator()(int j) const { return A_.at(size_t(j - 1)); }
+  ITensor &ref(int j) { return A_.at(size_t(j - 1)); }
+  int leftLim() const { return llim_; }
+  int rightLim() const { return rlim_; }
+  void leftLim(int x) { llim_ = x; }
+  void rightLim(int x) { rlim_ = x; }
+  std::vector<ITensor> const &tensors() const { return A_; }
+
+ private:
+  std::vector<ITensor> A_;
+  int llim_ = 0, rlim_ = 2;
+};
+
+using MPO = std::vector<ITensor>;  // W_1..W_N (0-based storage)
+
+// The two-site projected Hamiltonian acting on phi (mps/localop.h).
+class LocalOp {
+ public:
+  LocalOp() = default;
+  void update(ITensor const *Op1, ITensor const *Op2, ITensor const *L, ITensor const *R) {
+    Op1_ = Op1;
+    Op2_ = Op2;
+    L_ = L;
+    R_ = R;
+  }
+  bool LIsNull() const { return L_ == nullptr || !(*L_); }
+  bool RIsNull() const { return R_ == nullptr || !(*R_); }
+  // product (localop.h:324-365), two centre sites; contractions run in the
+  // precision form `prec` (ITX_PREC_*), everything else stays FP64
+  void product(ITensor const &phi, ITensor &phip) const {
+    PrecisionScope ps(prec_);
+    if (LIsNull()) {
+      phip = phi;
+      if (!RIsNull()) phip *= *R_;
+      phip *= *Op2_;
+      phip *= *Op1_;
+    } else {
+      phip = phi * (*L_);
+      phip *= *Op1_;
+      phip *= *Op2_;
+      if (!RIsNull()) phip *= *R_;
+    }
+    phip.noPrime();
+  }
+  long size() const;  // localop.h:488-530
+  void setPrecision(int p) { prec_ = p; }
+
+ private:
+  ITensor const *Op1_ = nullptr, *Op2_ = nullptr, *L_ = nullptr, *R_ = nullptr;
+  int prec_ = ITX_PREC_F64;
+};
+
+class LocalMPO {
+ public:
+  explicit LocalMPO(MPO const &H) : H_(&H), PH_(H.size() + 2), LHlim_(0), RHlim_(int(H.size()) + 1) {}
+  // position (localmpo.h:443-470) with makeL / makeR (:542-618)
+  void position(int b, MPS const &psi);
+  void product(ITensor const &phi, ITensor &phip) const { lop_.product(phi, phip); }
+  long size() const { return lop_.size(); }
+  ITensor const &L() const { return PH_.at(size_t(LHlim_)); }
+  ITensor const &R() const { return PH_.at(size_t(RHlim_)); }
+  void setPrecision(int p) { lop_.setPrecision(p); }
+  double envSeconds() const { return env_seconds_; }
+
+ private:
+  MPO const *H_;
+  std::vector<ITensor> PH_;
+  int LHlim_, RHlim_;
+  LocalOp lop_;
+  double env_seconds_ = 0;
+};
+
+struct SweepParams {
+  long maxdim = 1, mindim = 1, niter = 2;
+  double cutoff = 1e-8, noise = 0;
+};
+// Sweeps (mps/sweeps.h:42-145): one entry per sweep
+using Sweeps = std::vector<SweepParams>;
+
+inline void sweepnext(int &b, int &ha, int N) {
+  const int inc = (ha == 1 ? +1 : -1);
+  b += inc;
+  if (b == (ha == 1 ? N : 0)) {
+    b -= inc;
+    ++ha;
+  }
+}
+
+struct BondRecord {
+  int sweep, b, ha;
+  double energy, truncerr;
+  long dim;
+  int products;
+  double seconds;
+};
+
+// MPS::svdBond, density-matrix route (mps/mps_impl.h:21-99)
+Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args);
+
+// dmrg (mps/dmrg.h:51-76, 336-467).  obs (optional) receives one record
+// per bond step; sweep_seconds (optional) the wall time of each sweep.
+Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args,
+          std::function<void(BondRecord const &)> obs = nullptr, std::vector<double> *sweep_seconds = nullptr);
+
+}  // namespace itensor

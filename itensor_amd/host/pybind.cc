@@ -1,0 +1,204 @@
+// Python bindings of the ITensor-compatible host layer (tests, benchmark).
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "mps.h"
+
+namespace py = pybind11;
+using namespace itensor;
+
+namespace {
+
+using QNList = std::vector<std::pair<std::string, int>>;
+
+QN qnFrom(QNList const &l) {
+  QN q;
+  for (auto &p : l) q = q + QN(p.first, p.second);
+  return q;
+}
+
+QNList qnTo(QN const &q) { return q.store(); }
+
+Args argsFrom(py::dict const &d) {
+  Args a;
+  for (auto item : d) {
+    std::string k = py::str(item.first);
+    py::handle v = item.second;
+    if (py::isinstance<py::bool_>(v))
+      a.add(k, v.cast<bool>());
+    else if (py::isinstance<py::int_>(v))
+      a.add(k, v.cast<long>());
+    else if (py::isinstance<py::float_>(v))
+      a.add(k, v.cast<double>());
+    else if (py::isinstance<py::str>(v))
+      a.add(k, v.cast<std::string>());
+    else
+      throw ITError("unsupported Args value for key " + k);
+  }
+  return a;
+}
+
+Sweeps sweepsFrom(py::list const &l) {
+  Sweeps s;
+  for (auto item : l) {
+    py::dict d = item.cast<py::dict>();
+    SweepParams p;
+    p.maxdim = d["MaxDim"].cast<long>();
+    if (d.contains("MinDim")) p.mindim = d["MinDim"].cast<long>();
+    if (d.contains("Cutoff")) p.cutoff = d["Cutoff"].cast<double>();
+    if (d.contains("Niter")) p.niter = d["Niter"].cast<long>();
+    if (d.contains("Noise")) p.noise = d["Noise"].cast<double>();
+    s.push_back(p);
+  }
+  return s;
+}
+
+py::array_t<double> store_np(ITensor const &T) {
+  auto h = storageToHost(T);
+  py::array_t<double> a(h.size());
+  std::copy(h.begin(), h.end(), a.mutable_data());
+  return a;
+}
+
+py::dict recordDict(BondRecord const &r) {
+  py::dict d;
+  d["sweep"] = r.sweep;
+  d["b"] = r.b;
+  d["ha"] = r.ha;
+  d["energy"] = r.energy;
+  d["truncerr"] = r.truncerr;
+  d["dim"] = r.dim;
+  d["nprod"] = r.products;
+  d["seconds"] = r.seconds;
+  return d;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_itensor_host, m) {
+  m.doc() = "ITensor-compatible host layer of the MI355X DMRG H*psi engine";
+  py::register_exception<ITError>(m, "ITError");
+
+  py::class_<Index>(m, "Index")
+      .def(py::init([](uint64_t id, long dim, int dir, int plev, std::string const &tags,
+                       std::vector<std::pair<QNList, long>> const &qns) {
+             QNBlocks qb;
+             for (auto &p : qns) qb.push_back({qnFrom(p.first), p.second});
+             return Index(id, dim, Arrow(dir), plev, tags, qb);
+           }),
+           py::arg("id"), py::arg("dim"), py::arg("dir") = 1, py::arg("plev") = 0, py::arg("tags") = "",
+           py::arg("qns") = std::vector<std::pair<QNList, long>>())
+      .def_property_readonly("id", &Index::id)
+      .def_property_readonly("dim", [](Index const &i) { return i.dim(); })
+      .def_property_readonly("dir", [](Index const &i) { return int(i.dir()); })
+      .def_property_readonly("plev", &Index::primeLevel)
+      .def_property_readonly("tags", &Index::tags)
+      .def_property_readonly("qns",
+                             [](Index const &i) {
+                               std::vector<std::pair<QNList, long>> r;
+                               for (auto &p : i.qnBlocks()) r.push_back({qnTo(p.first), p.second});
+                               return r;
+                             })
+      .def("prime", [](Index i, int inc) { return i.prime(inc); }, py::arg("inc") = 1)
+      .def("dag", [](Index i) { return i.dag(); })
+      .def("noprime", [](Index i) { return i.noPrime(); })
+      .def("__eq__", [](Index const &a, Index const &b) { return a == b; })
+      .def("__repr__", &Index::str);
+
+  py::class_<ITensor>(m, "ITensor")
+      .def(py::init<>())
+      .def_property_readonly("inds", [](ITensor const &T) { return T.inds(); })
+      .def_property_readonly("is_qn", &ITensor::isQN)
+      .def_property_readonly("order", &ITensor::order)
+      .def("nnz", &ITensor::nnz)
+      .def("nnzblocks", &ITensor::nnzblocks)
+      .def_property_readonly("offsets",
+                             [](ITensor const &T) {
+                               std::vector<std::pair<std::vector<int>, int64_t>> r;
+                               for (auto &b : T.store().offsets) r.push_back({b.block, b.offset});
+                               return r;
+                             })
+      .def("store", &store_np)
+      .def("elt", &ITensor::elt)
+      .def("__bool__", [](ITensor const &T) { return bool(T); })
+      .def("__mul__", [](ITensor const &a, ITensor const &b) { return a * b; })
+      .def("__mul__", [](ITensor const &a, double r) { return a * r; })
+      .def("__rmul__", [](ITensor const &a, double r) { return a * r; })
+      .def("__add__", [](ITensor const &a, ITensor const &b) { return a + b; })
+      .def("__sub__", [](ITensor const &a, ITensor const &b) { return a - b; })
+      .def("dag", [](ITensor const &a) { return dag(a); })
+      .def("prime", [](ITensor const &a, int inc) { return prime(a, inc); }, py::arg("inc") = 1)
+      .def("prime_index", [](ITensor const &a, Index const &I, int inc) { return prime(a, I, inc); },
+           py::arg("I"), py::arg("inc") = 1)
+      .def("noprime", [](ITensor const &a) { return noPrime(a); });
+
+  m.def("dense", [](std::vector<Index> const &is, py::array_t<double, py::array::forcecast> a) {
+    std::vector<double> v(a.data(), a.data() + a.size());
+    return denseFromHost(is, v);
+  }, "Dense ITensor from a flat column-major array");
+  m.def("qdense", [](std::vector<Index> const &is, std::vector<std::pair<std::vector<int>, int64_t>> const &offs,
+                     py::array_t<double, py::array::forcecast> a) {
+    BlockOffsets bo;
+    for (auto &p : offs) bo.push_back({p.first, p.second});
+    std::vector<double> v(a.data(), a.data() + a.size());
+    return qdenseFromHost(is, bo, v);
+  }, "QDense ITensor from its block list and flat block store");
+  m.def("norm", &norm);
+  m.def("dot", &dotDag, "eltC(dag(A)*B)");
+  m.def("remove_qns", &removeQNs);
+  m.def("combiner", [](std::vector<Index> const &is, std::string const &tags) {
+    Args a;
+    a.add("Tags", tags);
+    return combiner(is, a);
+  }, py::arg("inds"), py::arg("tags") = "Link,CMB");
+  m.def("diag_hermitian", [](ITensor const &rho, py::dict args) {
+    ITensor U, D;
+    auto spec = diag_hermitian(rho, U, D, argsFrom(args));
+    return py::make_tuple(U, D, spec.eigs, spec.truncerr);
+  });
+  m.def("denmat_decomp", [](ITensor const &AA, ITensor A, ITensor B, bool fromleft, py::dict args) {
+    auto spec = denmatDecomp(AA, A, B, fromleft ? Fromleft : Fromright, argsFrom(args));
+    return py::make_tuple(A, B, spec.eigs, spec.truncerr);
+  });
+  m.def("truncate", [](std::vector<double> P, long maxdim, long mindim, double cutoff, bool absc, bool rel,
+                       bool respect) {
+    Args a;
+    a.add("RespectDegenerate", respect);
+    auto r = truncate(P, maxdim, mindim, cutoff, absc, rel, a);
+    return py::make_tuple(P, std::get<0>(r), std::get<1>(r), std::get<2>(r), std::get<3>(r));
+  });
+  m.def("davidson_bond", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, int b, py::dict args) {
+    MPO mpo(H);
+    MPS psi(A);
+    LocalMPO PH(mpo);
+    PH.position(b, psi);
+    ITensor phi = psi(b) * psi(b + 1);
+    DavidsonStats st;
+    double e = davidson(PH, phi, argsFrom(args), &st);
+    return py::make_tuple(e, phi, st.products);
+  }, "davidson on the two-site problem of bond b (iterativesolvers_test.cc:43-65 pattern)");
+  m.def("local_product", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, int b, ITensor const &phi,
+                            int prec) {
+    MPO mpo(H);
+    MPS psi(A);
+    LocalMPO PH(mpo);
+    PH.position(b, psi);
+    PH.setPrecision(prec);
+    ITensor out;
+    PH.product(phi, out);
+    return out;
+  }, py::arg("H"), py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("prec") = 0);
+  m.def("dmrg", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, py::list sweeps, py::dict args) {
+    MPS psi(A);
+    std::vector<py::dict> recs;
+    std::vector<double> secs;
+    double e = dmrg(psi, H, sweepsFrom(sweeps), argsFrom(args),
+                    [&](BondRecord const &r) { recs.push_back(recordDict(r)); }, &secs);
+    return py::make_tuple(e, recs, psi.tensors(), secs);
+  });
+  m.def("set_stream", [](uintptr_t s) { setStream(reinterpret_cast<void *>(s)); });
+  m.def("synchronize", []() { check(itx_synchronize(ctx()), "itx_synchronize"); });
+  m.def("set_contract_precision", &setContractPrecision);
+}

@@ -65,6 +65,8 @@ _PROTOS = {
     "itx_lincomb": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.c_int, _vp, C.c_size_t]),
     "itx_permute": (C.c_int, [_vp, _vp, _vp, C.c_int, _i64p, _i32p, C.c_double, C.c_double]),
     "itx_syev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "itx_syev_batched": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, _i64p, _vp]),
+    "itx_block_copy": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _i64p, C.c_double, C.c_double]),
 }
 
 
@@ -216,6 +218,19 @@ class Context:
 
     def syev(self, n: int, A: int, w: int):
         self.check(self.L.itx_syev(self.h, n, _vp(A), _vp(w)), "itx_syev")
+
+    def syev_batched(self, ns, a_offs, A: int, w_offs, w: int):
+        ns = np.ascontiguousarray(ns, dtype=np.int32)
+        ao = np.ascontiguousarray(a_offs, dtype=np.int64)
+        wo = np.ascontiguousarray(w_offs, dtype=np.int64)
+        self.check(self.L.itx_syev_batched(self.h, len(ns), ns.ctypes.data_as(_i32p), ao.ctypes.data_as(_i64p),
+                                           _vp(A), wo.ctypes.data_as(_i64p), _vp(w)), "itx_syev_batched")
+
+    def block_copy(self, X: int, Y: int, r: int, desc, alpha=1.0, beta=0.0):
+        desc = np.ascontiguousarray(desc, dtype=np.int64).ravel()
+        nt = len(desc) // (2 + 3 * r)
+        self.check(self.L.itx_block_copy(self.h, _vp(X), _vp(Y), nt, r, desc.ctypes.data_as(_i64p),
+                                         float(alpha), float(beta)), "itx_block_copy")
 
 
 class Plan:

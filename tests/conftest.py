@@ -4,6 +4,8 @@ import sys
 import numpy as np
 import pytest
 
+import itensor_amd  # noqa: F401  (imports torch before the native libraries)
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -1,0 +1,230 @@
+"""GPU parity of the ITensor-compatible C++ host layer (which calls libitx
+through the C ABI) against the oracle: contraction layouts and QN block
+patterns bit-exact, values to 1e-12, Davidson / denmatDecomp / DMRG
+energies against closed forms, exact diagonalisation and the oracle's
+own trajectory."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as OD
+from oracle.itensor import QN, Index, add_into, combiner, contract, norm, qdense, dense
+from tests.bridge import cidx, cten, oten, same_inds
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "heisenberg_ed.json")))
+
+
+@pytest.fixture(scope="module")
+def H(torch_dev):
+    from itensor_amd import host
+
+    return host
+
+
+def _rel(a, b):
+    den = max(np.max(np.abs(b)) if np.size(b) else 0.0, 1e-300)
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / den) if np.size(b) else 0.0
+
+
+def _qn_pair(seed):
+    rng = np.random.default_rng(seed)
+
+    def rind(tag):
+        nb = int(rng.integers(1, 5))
+        return Index([(QN(int(q)), int(rng.integers(1, 6))) for q in rng.integers(-2, 3, size=nb)], tag)
+
+    a, b, c, d = rind("a"), rind("b"), rind("c"), rind("d")
+    A = qdense([a, b.dag(), c], QN(int(rng.integers(-1, 2))), rng=rng)
+    B = qdense([c.dag(), d, b], QN(int(rng.integers(-1, 2))), rng=rng)
+    return A, B
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_qn_contraction_layout_bit_exact(H, seed):
+    A, B = _qn_pair(seed)
+    ref = contract(A, B)
+    C = cten(A) * cten(B)
+    assert same_inds(C, cten(ref))
+    assert [(tuple(b), o) for b, o in C.offsets] == [(b, o) for b, o in ref.offsets]
+    assert _rel(C.store(), ref.store) < 1e-12
+
+
+def test_dense_contraction_and_scalar(H):
+    rng = np.random.default_rng(3)
+    i, j, k = Index(7, "i"), Index(5, "j"), Index(9, "k")
+    A = dense([i, j, k], rng.standard_normal((7, 5, 9)))
+    B = dense([k, i], rng.standard_normal((9, 7)))
+    ref = contract(A, B)
+    C = cten(A) * cten(B)
+    assert same_inds(C, cten(ref))
+    assert _rel(C.store(), ref.d.ravel(order="F")) < 1e-12
+    full = contract(A, A.dag())
+    assert abs((cten(A) * cten(A).dag()).elt() - full.elt()) < 1e-10
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_qn_add_union_and_permuted(H, seed):
+    rng = np.random.default_rng(50 + seed)
+    i = Index([(QN(0), 2), (QN(1), 3), (QN(2), 4), (QN(1), 5), (QN(3), 6)], "i")
+    l = Index([(QN(0), 3)], "l")
+    a = qdense([i, l], QN(1), rng=rng)
+    A = contract(a, a.dag().prime(1, i))          # block deficient (4 blocks)
+    T = qdense([i.prime().dag(), i], QN(0), rng=rng)  # full, permuted index order
+    ref = add_into(A.copy(), T, -0.5)
+    got = cten(A) + (cten(T) * -0.5)
+    assert got.nnzblocks() == ref.nnzblocks() == 7
+    assert [(tuple(b), o) for b, o in got.offsets] == [(b, o) for b, o in ref.offsets]
+    assert _rel(got.store(), ref.store) < 1e-14
+
+
+def test_dense_add_permuted(H):
+    rng = np.random.default_rng(9)
+    i, j, k = Index(3, "i"), Index(4, "j"), Index(5, "k")
+    A = dense([i, j, k], rng.standard_normal((3, 4, 5)))
+    B = dense([k, i, j], rng.standard_normal((5, 3, 4)))
+    ref = add_into(A.copy(), B, 2.0)
+    got = cten(A) + cten(B) * 2.0
+    assert _rel(got.store(), ref.d.ravel(order="F")) < 1e-14
+
+
+def test_qn_combiner_matches_oracle(H):
+    rng = np.random.default_rng(5)
+    l = Index([(QN(0), 2), (QN(1), 3), (QN(-1), 1)], "l")
+    s = Index([(QN(1), 1), (QN(-1), 1)], "s")
+    r = Index([(QN(0), 2), (QN(1), 2)], "r")
+    T = qdense([l, s, r.dag()], QN(0), rng=rng)
+    Co, co = combiner([l, s])
+    ref = Co * T
+    Cc, cc = H.combiner([cidx(l), cidx(s)], "Link,CMB")
+    assert [(list(q), m) for q, m in cc.qns] == [(list(q.q), m) for q, m in co.qns]
+    got = Cc * cten(T)
+    assert [(tuple(b), o) for b, o in got.offsets] == [(b, o) for b, o in ref.offsets]
+    assert _rel(got.store(), ref.store) < 1e-15
+    back = Cc.dag() * got
+    refb = Co.dag() * ref
+    assert [(tuple(b), o) for b, o in back.offsets] == [(b, o) for b, o in refb.offsets]
+    assert _rel(back.store(), refb.store) < 1e-15
+
+
+def test_dense_combiner_permuting(H):
+    rng = np.random.default_rng(6)
+    i, j, k = Index(2, "i"), Index(3, "j"), Index(4, "k")
+    T = dense([i, j, k], rng.standard_normal((2, 3, 4)))
+    Co, co = combiner([k, i])
+    ref = Co * T
+    Cc, cc = H.combiner([cidx(k), cidx(i)], "Link,CMB")
+    got = Cc * cten(T)
+    assert [x.dim for x in got.inds] == [x.dim for x in ref.inds]
+    assert _rel(got.store(), ref.d.ravel(order="F")) < 1e-15
+
+
+def test_diag_hermitian_qn_sectors_exact(H):
+    rng = np.random.default_rng(4)
+    j = Index([(QN(0), 3), (QN(1), 4), (QN(-1), 2)], "j")
+    a = qdense([j, Index([(QN(0), 2), (QN(-1), 3), (QN(1), 2)], "k").dag()], QN(0), rng=rng)
+    rho = a * a.dag().prime(1, j)
+    args = {"MaxDim": 6, "Cutoff": 1e-12, "RespectDegenerate": True}
+    Uo, Do, so = OD.diag_hermitian(rho, dict(args))
+    U, D, eigs, te = H.diag_hermitian(cten(rho), args)
+    assert np.allclose(eigs, so["eigs"], rtol=1e-12, atol=1e-14)
+    newo = Uo.inds[1]
+    newc = U.inds[1]
+    assert [(list(q), m) for q, m in newc.qns] == [(list(q.q), m) for q, m in newo.qns]
+    assert [(tuple(b), o) for b, o in U.offsets] == [(b, o) for b, o in Uo.offsets]
+    # columns agree up to sign
+    Ug, Ur = oten(U).to_dense_array(), Uo.to_dense_array()
+    for c in range(Ur.shape[1]):
+        assert min(np.max(np.abs(Ug[:, c] - Ur[:, c])), np.max(np.abs(Ug[:, c] + Ur[:, c]))) < 1e-10
+
+
+def test_qn_denmat_decomp_reconstructs(H):
+    """decomp_test.cc:585-625 on the GPU path."""
+    rng = np.random.default_rng(5)
+    S1 = Index([(QN(1), 1), (QN(-1), 1)], "S1")
+    S2 = Index([(QN(1), 1), (QN(-1), 1)], "S2")
+    L1 = Index([(QN(2), 3), (QN(1), 4), (QN(0), 8), (QN(-1), 4), (QN(-2), 2)], "L1")
+    L2 = Index([(QN(2), 4), (QN(1), 6), (QN(0), 10), (QN(-1), 4), (QN(-2), 3)], "L2")
+    L3 = Index([(QN(2), 2), (QN(0), 4), (QN(-2), 2)], "L3")
+    A1 = qdense([L1, S1, L2], QN(), rng=rng)
+    A2 = qdense([L2.dag(), S2, L3], QN(), rng=rng)
+    AA = A1 * A2
+    AA = AA * (-1.0 / norm(AA))
+    cA, cB, eigs, te = H.denmat_decomp(cten(AA), cten(A1), cten(A2), True, {"Truncate": True})
+    res = cA * cB
+    diff = cten(AA) - res
+    assert H.norm(diff) < 1e-11
+    assert np.all(np.asarray(eigs) >= 0)
+
+
+@pytest.mark.parametrize("qn", [False, True])
+def test_davidson_closed_form_gpu(H, qn):
+    """iterativesolvers_test.cc:43-65 through the C++ LocalMPO + davidson."""
+    sites = OD.spin_half_sites(4, qn)
+    Hm = OD.heisenberg_mpo(sites)
+    psi = OD.product_mps(sites, [0, 1, 0, 1])
+    e, phi, nprod = H.davidson_bond([cten(w) for w in Hm], [cten(a) for a in psi.A], 2, {"MaxIter": 9})
+    assert abs(e - (-(0.25 + 1 / math.sqrt(2)))) < 1e-10
+
+
+def test_local_product_matches_oracle_dense(H):
+    sites = OD.spin_half_sites(8, False)
+    Hm = OD.heisenberg_mpo(sites)
+    psi = OD.random_mps(sites, 16, seed=11)
+    PH = OD.LocalMPO(Hm)
+    for b in (1, 4, 7):
+        PH.position(b, psi)
+        phi = psi(b) * psi(b + 1)
+        ref = PH.product(phi)
+        got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi))
+        assert same_inds(got, cten(ref))
+        assert _rel(got.store(), ref.d.ravel(order="F")) < 1e-12
+
+
+def test_dmrg_config1_trajectory_matches_oracle(H):
+    """Config 1 (N=20 Heisenberg, random MPS chi=10, maxdim
+    [10,20,100,100,200], cutoff 1e-10, niter 2): GPU FP64 vs the oracle on
+    the same initial state: per-bond energies to 1e-10, the same kept
+    dimensions, final energy vs exact diagonalisation."""
+    N = 20
+    sites = OD.spin_half_sites(N, False)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.random_mps(sites, 10, seed=20261202 + 1)
+    sw = [{"MaxDim": m, "Cutoff": 1e-10, "Niter": 2} for m in (10, 20, 100, 100, 200)]
+    trace = []
+    Eo, _ = OD.dmrg(Hm, psi0, sw, record=trace.append)
+    Eg, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {})
+    assert len(recs) == len(trace) == 5 * 2 * (N - 1)
+    assert [r["dim"] for r in recs] == [t["dim"] for t in trace]
+    assert max(abs(r["energy"] - t["energy"]) for r, t in zip(recs, trace)) < 1e-10
+    assert abs(Eg - Eo) < 1e-10
+    assert abs(Eg - GOLD["energies"]["20"]) < 1e-8
+    print(f"config-1 sec/sweep (GPU FP64): {secs}")
+
+
+def test_qn_dmrg_matches_oracle_and_ed(H):
+    N = 12
+    sites = OD.spin_half_sites(N, True)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)])
+    sw = [{"MaxDim": m, "Cutoff": 1e-12, "Niter": 2} for m in (10, 20, 40, 60, 60)]
+    trace = []
+    Eo, _ = OD.dmrg(Hm, psi0, sw, record=trace.append)
+    Eg, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {})
+    assert [r["dim"] for r in recs] == [t["dim"] for t in trace]
+    assert max(abs(r["energy"] - t["energy"]) for r, t in zip(recs, trace)) < 1e-10
+    assert abs(Eg - GOLD["energies"]["12"]) < 1e-9
+
+
+def test_tfim_n32_closed_form_gpu(H):
+    N = 32
+    sites = OD.spin_half_sites(N, False)
+    Hm = OD.tfim_mpo(sites, 0.5)
+    psi0 = OD.product_mps(sites, [0] * N)
+    sw = [{"MaxDim": m, "Cutoff": 1e-12} for m in (10, 20, 30, 30, 30)]
+    E, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {})
+    exact = (1.0 - 1.0 / math.sin(math.pi / (2 * (2 * N + 1)))) / 4
+    assert abs((E - exact) / exact) < 1e-5
