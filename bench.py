@@ -5,13 +5,16 @@ H*psi = noPrime(((psi*L)*W1)*W2*R) with psi in R^{chi x 2 x 2 x chi},
 L, R in R^{chi x 5 x chi}, W1 = W2 = the bulk Heisenberg MPO tensor
 (5 x 2 x 2 x 5), chi = 1024, entries of psi, L, R i.i.d. N(0,1) from a fixed
 seed (synthetic data), all operands resident in HBM before timing.
-The four contractions run through the libitx C ABI (plans built once).
+The contractions run through the libitx C ABI (plans built once), in the
+order LocalOp::product runs them: psi*L, then the MPO pair pre-contracted
+(W12 = W1*W2, same flop count and result layout), then *R (--chain 4stage
+runs the reference's four separate stages instead).
 
 value = algorithmic FP64-equivalent flops (80 chi^3 + 800 chi^2 per H*psi)
         / wall time, whole job (one GPU).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 1024]
-                       [--no-cpu] [--prec f64|f32]
+                       [--no-cpu] [--prec f64|f32] [--chain fused|4stage]
 """
 from __future__ import annotations
 
@@ -48,6 +51,22 @@ STAGES = [
     ((-1, 3, 4, 5, -2), (-2, -1, 6, 7), (3, 4, 5, 6, 7)),
     ((-1, 3, 4, 5, -2), (-1, -2, 6), (3, 4, 5, 6)),
 ]
+# fused chain of LocalOp::product: S1, then T1 x W12 with W12 = W1*W2
+# (h0,s1,s1',s2,s2',h2), then S4 -- same result layout and flop count
+FUSED = [
+    STAGES[0],
+    ((-1, -2, 3, -3, 4), (-3, -1, 5, -2, 6, 7), (3, 4, 5, 6, 7)),
+    STAGES[3],
+]
+
+
+def fused_dims(chi: int, d: int = 2, k: int = 5):
+    psi = (chi, d, d, chi)
+    Ld = (chi, k, chi)
+    T1 = (d, d, chi, k, chi)
+    W12 = (k, d, d, d, d, k)
+    T3 = (chi, chi, d, d, k)
+    return [(psi, Ld, T1), (T1, W12, T3), (T3, Ld, psi)]
 
 
 def heisenberg_w() -> np.ndarray:
@@ -99,6 +118,8 @@ def main():
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chain", default="fused", choices=["fused", "4stage"],
+                    help="LocalOp::product as run by the host layer (MPO pair fused) or the 4-stage chain")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
@@ -115,6 +136,7 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     ctx = Context(0, stream.cuda_stream)
+    ctx.set_autotune(True)  # tile shapes measured per size class at plan creation (untimed)
 
     chi = args.chi
     rng = np.random.default_rng(SEED)
@@ -127,14 +149,20 @@ def main():
         return torch.from_numpy(np.ascontiguousarray(a.ravel(order="F"))).to(dev)
 
     d_psi, d_L, d_R, d_W = put(psi), put(L), put(R), put(W)
-    dims = stage_dims(chi)
+    if args.chain == "fused":
+        W12 = np.einsum("aijb,bklc->aijklc", W, W)  # W1*W2 over h1: (h0,s1,s1',s2,s2',h2)
+        dims, labels = fused_dims(chi), FUSED
+        ops = [d_L, put(W12), d_R]
+    else:
+        dims, labels = stage_dims(chi), STAGES
+        ops = [d_L, d_W, d_W, d_R]
+    nst = len(dims)
     bufs = [torch.empty(int(np.prod(dc)), dtype=torch.float64, device=dev) for (_, _, dc) in dims]
-    ops = [d_L, d_W, d_W, d_R]
-    inputs = [d_psi, bufs[0], bufs[1], bufs[2]]
+    inputs = [d_psi] + bufs[:-1]
 
     def run_forms(prec):
         plans = [ctx.plan_dense(da, la, db, lb, dc, lc, prec)
-                 for (da, db, dc), (la, lb, lc) in zip(dims, STAGES)]
+                 for (da, db, dc), (la, lb, lc) in zip(dims, labels)]
         return plans
 
     def hpsi(plans):
@@ -155,16 +183,16 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         # per-stage HIP-event timing on the stream the kernels run on
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nst + 1)] for _ in range(args.steps)]
         for s in range(args.steps):
             for i, (p, a, b, c) in enumerate(zip(plans, inputs, ops, bufs)):
                 ev[s][i].record(stream)
                 p.execute(a.data_ptr(), b.data_ptr(), c.data_ptr())
-            ev[s][4].record(stream)
+            ev[s][nst].record(stream)
         torch.cuda.synchronize()
         stage_ms = [float(np.mean([ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)]))
-                    for i in range(4)]
-        out = bufs[3].detach().cpu().numpy().reshape((chi, 2, 2, chi), order="F")
+                    for i in range(nst)]
+        out = bufs[-1].detach().cpu().numpy().reshape((chi, 2, 2, chi), order="F")
         results[prec] = {"wall_s": wall, "stage_ms": stage_ms, "out": out,
                          "stage_flops": [p.flops for p in plans]}
         for p in plans:
@@ -194,10 +222,10 @@ def main():
         "dtype": args.prec,
         "data": "synthetic (psi, L, R iid N(0,1), seed %d; Heisenberg W)" % SEED,
         "config": {"workload": "dense two-site Hpsi, chi=%d, d=2, k=5 (Heisenberg MPO)" % chi,
-                   "chi": chi, "flops_per_step": flops, "parallelism": "single GPU"},
+                   "chi": chi, "flops_per_step": flops, "chain": args.chain, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK[args.prec],
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
-                     "kernel": "k_gemm (stage S%d of H*psi)" % (dom + 1),
+                     "kernel": "k_gemm (stage %d of %d of H*psi)" % (dom + 1, nst),
                      "flops_per_launch": dom_flops, "avg_launch_ms": round(st[dom], 4)},
         "stages_ms": {args.prec: [round(x, 4) for x in st]},
         "forms": {},

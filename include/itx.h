@@ -132,6 +132,17 @@ int itx_plan_qn(itx_ctx *ctx,
 int itx_plan_execute(itx_ctx *ctx, const itx_plan *plan,
                      const double *A, const double *B, double *C,
                      double alpha, double beta);
+/* Tile autotuner.  With autotuning on, a plan whose size class (precision,
+ * operand layout, log2 buckets of block shape, summed K and block count)
+ * has no entry in the context's tuning table times every MFMA tile shape
+ * on scratch buffers and records the fastest; later plans of that class
+ * use the table.  Plans must not be created under graph capture while
+ * autotuning is on.  The table can be saved and loaded as text. */
+int itx_set_autotune(itx_ctx *ctx, int on);
+int itx_tune_save(itx_ctx *ctx, const char *path);
+int itx_tune_load(itx_ctx *ctx, const char *path);
+int itx_tune_size(itx_ctx *ctx);
+
 /* Algorithmic FP64-equivalent flops of one execution (sum of 2*m*n*k). */
 double itx_plan_flops(const itx_plan *plan);
 int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan);

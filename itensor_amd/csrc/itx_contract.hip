@@ -16,9 +16,14 @@
 //   OpenMP path groups by output block for the same reason,
 //   itdata/qutil.h:271-349).
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <queue>
 #include <sstream>
 
 #include "itx_internal.h"
@@ -26,7 +31,6 @@
 namespace itx {
 namespace {
 
-constexpr int NT = 256;  // threads per workgroup (4 wave64)
 constexpr int BK = 16;   // K depth of one LDS stage
 
 template <typename T>
@@ -60,9 +64,11 @@ struct MF<float> {
 
 // Loader of one operand tile (rows x BK) into LDS laid out [k][row].
 // KC: the operand is contiguous along k (lanes walk k), else along rows.
-template <typename T, int BR, bool KC>
+// Offsets stay 64-bit: 32-bit offsets measured 1.6x slower on gfx950.
+template <typename T, int BR, bool KC, int NTH>
 struct TileLoader {
-  static constexpr int PER = BR * BK / NT;  // elements per thread
+  static constexpr int PER = BR * BK / NTH;  // elements per thread
+  static_assert(PER >= 1, "tile too small for the thread count");
   static constexpr int LD = BR + MF<T>::template pad<BR>();
   double r[PER];
   int64_t roff[KC ? PER : 1];
@@ -75,7 +81,7 @@ struct TileLoader {
       lr = tid / BK;
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
-        int row = row0 + lr + (NT / BK) * p;
+        int row = row0 + lr + (NTH / BK) * p;
         rv[p] = row < R;
         roff[p] = rv[p] ? tabR[row] : 0;
       }
@@ -98,7 +104,7 @@ struct TileLoader {
     } else {
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
-        int k = k0 + lk + (NT / BR) * p;
+        int k = k0 + lk + (NTH / BR) * p;
         bool kv = k < K;
         int64_t ko = kv ? tabK[k] : 0;
         r[p] = (kv && rv[0]) ? base[roff[0] + ko] : 0.0;
@@ -109,33 +115,37 @@ struct TileLoader {
     if constexpr (KC) {
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
-        int row = lr + (NT / BK) * p;
+        int row = lr + (NTH / BK) * p;
         s[lk * LD + (row ^ MF<T>::sw(lk))] = T(r[p]);
       }
     } else {
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
-        int k = lk + (NT / BR) * p;
+        int k = lk + (NTH / BR) * p;
         s[k * LD + (lr ^ MF<T>::sw(k))] = T(r[p]);
       }
     }
   }
 };
 
-template <typename T, int BM, int BN, bool AKC, bool BKC>
-__global__ __launch_bounds__(NT) void k_gemm(const double *__restrict__ A, const double *__restrict__ B,
-                                             double *__restrict__ C, const GemmTask *__restrict__ tasks,
-                                             const WorkUnit *__restrict__ units, int nunits,
-                                             const int64_t *__restrict__ tab, double alpha, double beta) {
+// One workgroup = GM x GN waves computing a BM x BN tile of C; each wave
+// owns a (BM/GM) x (BN/GN) sub-tile of 16x16 MFMA fragments.
+template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC>
+__global__ __launch_bounds__(64 * GM * GN) void k_gemm(const double *__restrict__ A, const double *__restrict__ B,
+                                                       double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                       const WorkUnit *__restrict__ units, int nunits,
+                                                       const int64_t *__restrict__ tab, double alpha, double beta) {
+  constexpr int NTH = 64 * GM * GN;
   using v4 = typename MF<T>::v4;
-  using LA = TileLoader<T, BM, AKC>;
-  using LB = TileLoader<T, BN, BKC>;
-  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  using LA = TileLoader<T, BM, AKC, NTH>;
+  using LB = TileLoader<T, BN, BKC, NTH>;
+  constexpr int WM = BM / GM, WN = BN / GN, FM = WM / 16, FN = WN / 16;
+  static_assert(FM >= 1 && FN >= 1, "wave tile below one MFMA fragment");
   __shared__ T As[2][BK * LA::LD];
   __shared__ T Bs[2][BK * LB::LD];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % GM, wn = wave / GM;
 
   // Units are spread so that each XCD (blocks b with equal b % 8 land on
   // the same XCD) walks a contiguous chunk of the unit list: neighbouring
@@ -238,6 +248,67 @@ __global__ __launch_bounds__(NT) void k_gemm(const double *__restrict__ A, const
             if (beta != 0.0) v += beta * Cb[ad];
             Cb[ad] = v;
           }
+        }
+      }
+    }
+  }
+}
+
+// Skinny contractions (N <= 32 and K <= 64 for every task, e.g. the
+// MPO-tensor stages of H*psi): HBM-bound, so one thread owns one row m of
+// a 256-row unit and accumulates its N outputs in registers (FP64 FMA)
+// while B (K x N) sits in LDS.  Same task/unit/table format as k_gemm.
+constexpr int SK_ROWS = 256, SK_N = 32, SK_K = 64;
+
+__global__ __launch_bounds__(SK_ROWS) void k_skinny(const double *__restrict__ A, const double *__restrict__ B,
+                                                    double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                    const WorkUnit *__restrict__ units, int nunits,
+                                                    const int64_t *__restrict__ tab, double alpha, double beta) {
+  __shared__ double Bs[SK_K * SK_N];
+  const int tid = threadIdx.x;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const WorkUnit wu = units[u];
+    const int m = wu.m0 + tid;
+    double acc[SK_N];
+#pragma unroll
+    for (int n = 0; n < SK_N; ++n) acc[n] = 0.0;
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      const int N = tk.N, K = tk.K;
+      __syncthreads();  // the previous task's B tile is no longer read
+      for (int x = tid; x < K * N; x += SK_ROWS) {
+        const int k = x / N, n = x - k * N;
+        Bs[k * SK_N + n] = B[tk.b_off + tab[tk.tBK + k] + tab[tk.tBN + n]];
+      }
+      __syncthreads();
+      if (m < tk.M) {
+        const double *Ar = A + tk.a_off + tab[tk.tAM + m];
+        const int64_t *tAK = tab + tk.tAK;
+        for (int k0 = 0; k0 < K; k0 += 8) {
+          double av[8];  // issue 8 independent loads before using any
+#pragma unroll
+          for (int j = 0; j < 8; ++j) av[j] = (k0 + j < K) ? Ar[tAK[k0 + j]] : 0.0;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            if (k0 + j < K) {
+#pragma unroll
+              for (int n = 0; n < SK_N; ++n)
+                if (n < N) acc[n] += av[j] * Bs[(k0 + j) * SK_N + n];
+            }
+          }
+        }
+      }
+    }
+    const GemmTask t0 = tasks[wu.tbeg];
+    if (m < t0.M) {
+      double *Cm = C + t0.c_off + tab[t0.tCM + m];
+      const int64_t *tCN = tab + t0.tCN;
+#pragma unroll
+      for (int n = 0; n < SK_N; ++n) {
+        if (n < t0.N) {
+          double v = alpha * acc[n];
+          if (beta != 0.0) v += beta * Cm[tCN[n]];
+          Cm[tCN[n]] = v;
         }
       }
     }
@@ -356,40 +427,137 @@ void colmajor_strides(int r, const int64_t *d, int64_t *s) {
   }
 }
 
-// Tile shapes of the engine (square tiles; index stored in the plan).
-constexpr int kTiles[3] = {128, 64, 32};
+// MFMA tiles (BM x BN with GM x GN waves) and the skinny VALU path.
+struct TileCfg {
+  int bm, bn, gm, gn;
+  double eff64, eff32;  // relative per-CU efficiency (MI355X measurements)
+};
+constexpr TileCfg kTiles[] = {
+    {128, 128, 2, 2, 0.55, 0.70},  // 0
+    {128, 64, 2, 2, 0.92, 1.00},   // 1
+    {64, 128, 2, 2, 0.92, 0.95},   // 2
+    {64, 64, 2, 2, 1.00, 0.95},    // 3
+    {32, 32, 2, 2, 0.50, 0.45},    // 4
+    {SK_ROWS, SK_N, 0, 0, 1.0, 1.0},  // 5: skinny
+    {128, 128, 2, 4, 0.80, 0.85},  // 6: 8 waves
+    {256, 64, 4, 2, 0.70, 0.55},   // 7: 8 waves
+    {64, 256, 2, 4, 0.95, 1.00},   // 8: 8 waves
+    {256, 128, 4, 2, 0.60, 0.50},  // 9: 8 waves, wave tile 64x64
+};
+constexpr int kNumTiles = 10;
+constexpr int kSkinny = 5;
+constexpr int kSkinnyOcc = 4;
 
-template <typename T, int BM, bool AKC, bool BKC>
+template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC>
 void launch_t(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C, double alpha,
               double beta) {
-  hipLaunchKernelGGL((k_gemm<T, BM, BM, AKC, BKC>), dim3(p->grid), dim3(NT), 0, s, A, B, C, p->d_tasks,
-                     p->d_units, p->nunits, p->d_tab, alpha, beta);
+  hipLaunchKernelGGL((k_gemm<T, BM, BN, GM, GN, AKC, BKC>), dim3(p->grid), dim3(64 * GM * GN), 0, s, A, B, C,
+                     p->d_tasks, p->d_units, p->nunits, p->d_tab, alpha, beta);
 }
 
-template <typename T, int BM>
+template <typename T, int BM, int BN, int GM, int GN>
 void launch_bm(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C, double alpha,
                double beta) {
   if (p->a_kcontig) {
     if (p->b_kcontig)
-      launch_t<T, BM, true, true>(p, s, A, B, C, alpha, beta);
+      launch_t<T, BM, BN, GM, GN, true, true>(p, s, A, B, C, alpha, beta);
     else
-      launch_t<T, BM, true, false>(p, s, A, B, C, alpha, beta);
+      launch_t<T, BM, BN, GM, GN, true, false>(p, s, A, B, C, alpha, beta);
   } else {
     if (p->b_kcontig)
-      launch_t<T, BM, false, true>(p, s, A, B, C, alpha, beta);
+      launch_t<T, BM, BN, GM, GN, false, true>(p, s, A, B, C, alpha, beta);
     else
-      launch_t<T, BM, false, false>(p, s, A, B, C, alpha, beta);
+      launch_t<T, BM, BN, GM, GN, false, false>(p, s, A, B, C, alpha, beta);
   }
 }
 
 template <typename T>
 void launch_prec(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C,
                  double alpha, double beta) {
-  switch (kTiles[p->tile]) {
-    case 128: launch_bm<T, 128>(p, s, A, B, C, alpha, beta); break;
-    case 64: launch_bm<T, 64>(p, s, A, B, C, alpha, beta); break;
-    default: launch_bm<T, 32>(p, s, A, B, C, alpha, beta); break;
+  switch (p->tile) {
+    case 0: launch_bm<T, 128, 128, 2, 2>(p, s, A, B, C, alpha, beta); break;
+    case 1: launch_bm<T, 128, 64, 2, 2>(p, s, A, B, C, alpha, beta); break;
+    case 2: launch_bm<T, 64, 128, 2, 2>(p, s, A, B, C, alpha, beta); break;
+    case 3: launch_bm<T, 64, 64, 2, 2>(p, s, A, B, C, alpha, beta); break;
+    case 6: launch_bm<T, 128, 128, 2, 4>(p, s, A, B, C, alpha, beta); break;
+    case 7: launch_bm<T, 256, 64, 4, 2>(p, s, A, B, C, alpha, beta); break;
+    case 8: launch_bm<T, 64, 256, 2, 4>(p, s, A, B, C, alpha, beta); break;
+    case 9: launch_bm<T, 256, 128, 4, 2>(p, s, A, B, C, alpha, beta); break;
+    default: launch_bm<T, 32, 32, 2, 2>(p, s, A, B, C, alpha, beta); break;
   }
+}
+
+// Resident workgroups per CU of one kernel instantiation (registers and
+// LDS), from the runtime's occupancy calculator; cached per variant.
+template <typename T, int BM, int BN, int GM, int GN>
+int occ_bm(bool akc, bool bkc) {
+  const void *f;
+  if (akc)
+    f = bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true>)
+            : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false>);
+  else
+    f = bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true>)
+            : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false>);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * GM * GN, 0) != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  return nb;
+}
+
+template <typename T>
+int occ_prec(int tile, bool akc, bool bkc) {
+  switch (tile) {
+    case 0: return occ_bm<T, 128, 128, 2, 2>(akc, bkc);
+    case 1: return occ_bm<T, 128, 64, 2, 2>(akc, bkc);
+    case 2: return occ_bm<T, 64, 128, 2, 2>(akc, bkc);
+    case 3: return occ_bm<T, 64, 64, 2, 2>(akc, bkc);
+    case 6: return occ_bm<T, 128, 128, 2, 4>(akc, bkc);
+    case 7: return occ_bm<T, 256, 64, 4, 2>(akc, bkc);
+    case 8: return occ_bm<T, 64, 256, 2, 4>(akc, bkc);
+    case 9: return occ_bm<T, 256, 128, 4, 2>(akc, bkc);
+    default: return occ_bm<T, 32, 32, 2, 2>(akc, bkc);
+  }
+}
+
+int tile_occupancy(int tile, int prec, bool akc, bool bkc) {
+  static std::map<int, int> cache;
+  if (tile == kSkinny) return kSkinnyOcc;
+  const int key = tile * 16 + prec * 4 + (akc ? 2 : 0) + (bkc ? 1 : 0);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int o = (prec == ITX_PREC_F64) ? occ_prec<double>(tile, akc, bkc) : occ_prec<float>(tile, akc, bkc);
+  cache[key] = o;
+  return o;
+}
+
+// Makespan of units (costs) on `slots` identical workers, heaviest first
+// onto the least loaded worker (what the persistent loop approximates).
+double lpt_makespan(std::vector<double> costs, int64_t slots) {
+  if (costs.empty()) return 0;
+  std::sort(costs.begin(), costs.end(), std::greater<double>());
+  if (int64_t(costs.size()) <= slots) return costs.front();
+  std::priority_queue<double, std::vector<double>, std::greater<double>> q;
+  for (int64_t i = 0; i < slots; ++i) q.push(0.0);
+  double mk = 0;
+  for (double c : costs) {
+    double t = q.top() + c;
+    q.pop();
+    q.push(t);
+    mk = std::max(mk, t);
+  }
+  return mk;
+}
+
+// Size class of a contraction for the tuning table: precision, operand
+// contiguity, and log2 buckets of the largest C block, summed K, #blocks.
+std::string tune_key(int prec, bool akc, bool bkc, int maxM, int maxN, double maxKsum, int ngroups) {
+  auto lg = [](double x) { return int(std::ceil(std::log2(std::max(1.0, x)))); };
+  char b[96];
+  std::snprintf(b, sizeof b, "p%d a%d b%d M%d N%d K%d G%d", prec, int(akc), int(bkc), lg(maxM), lg(maxN),
+                lg(maxKsum), lg(ngroups));
+  return b;
 }
 
 }  // namespace
@@ -447,6 +615,14 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX)
       return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: GEMM extent exceeds 2^31");
     if (M == 0 || N == 0) continue;  // empty C block: nothing to write
+    {
+      int64_t na = 1, nb = 1, nc = 1;
+      for (int i = 0; i < rA; ++i) na *= dA[i];
+      for (int j = 0; j < rB; ++j) nb *= dB[j];
+      for (int c = 0; c < rC; ++c) nc *= dC[c];
+      if (na > INT32_MAX || nb > INT32_MAX || nc > INT32_MAX)
+        return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: a block of 2^31 or more elements");
+    }
     if (!kc_set) {
       kc_set = true;
       akc = !dk.empty() && dk[0].s1 == 1;
@@ -498,96 +674,181 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     i = e;
   }
 
-  // pick the tile shape: estimated time = max(total/parallel, largest unit)/eff
-  const int cus = std::max(1, ctx->num_cus);
-  const double eff[3] = {1.0, 0.8, 0.55};
-  int best = 0;
-  double best_t = 1e300;
-  for (int ti = 0; ti < 3; ++ti) {
-    const int bt = kTiles[ti];
-    double tot = 0, mx = 0;
-    int64_t nu = 0;
-    for (auto &g : groups) {
-      int64_t tiles = int64_t((g.M + bt - 1) / bt) * ((g.N + bt - 1) / bt);
-      double c = double(bt) * bt * (g.ksum + BK);
-      nu += tiles;
-      tot += c * tiles;
-      mx = std::max(mx, c);
-    }
-    if (nu == 0) continue;
-    double tt = std::max(tot / double(std::min<int64_t>(nu, cus)), mx) / eff[ti];
-    if (tt < best_t * 0.97) {
-      best_t = tt;
-      best = ti;
-    }
-  }
-  const int bt = kTiles[best];
-  std::vector<WorkUnit> units;
-  std::vector<double> ucost;
-  for (auto &g : groups) {
-    for (int m0 = 0; m0 < g.M; m0 += bt)
-      for (int n0 = 0; n0 < g.N; n0 += bt) {
-        units.push_back({g.tb, g.te, m0, n0});
-        ucost.push_back(g.ksum);
-      }
-  }
-  // heaviest units first (the persistent loop then balances like LPT);
-  // stable so equal-cost tiles keep their m-major order
-  {
-    std::vector<int> ui(units.size());
-    std::iota(ui.begin(), ui.end(), 0);
-    std::stable_sort(ui.begin(), ui.end(), [&](int x, int y) { return ucost[x] > ucost[y]; });
-    std::vector<WorkUnit> su(units.size());
-    for (size_t i = 0; i < ui.size(); ++i) su[i] = units[ui[i]];
-    units.swap(su);
+  // operand extents (elements) for the autotuner's scratch buffers
+  int64_t extA = 1, extB = 1, extC = 1;
+  for (auto &t : st) {
+    auto mx = [&](int64_t start, int64_t n) {
+      int64_t v = 0;
+      for (int64_t i = 0; i < n; ++i) v = std::max(v, tab[size_t(start + i)]);
+      return v;
+    };
+    extA = std::max(extA, t.a_off + mx(t.tAM, t.M) + mx(t.tAK, t.K) + 1);
+    extB = std::max(extB, t.b_off + mx(t.tBK, t.K) + mx(t.tBN, t.N) + 1);
+    extC = std::max(extC, t.c_off + mx(t.tCM, t.M) + mx(t.tCN, t.N) + 1);
   }
 
-  auto *p = new itx_plan();
-  p->prec = prec;
-  p->tile = best;
-  p->a_kcontig = akc;
-  p->b_kcontig = bkc;
-  p->ntasks = int(st.size());
-  p->nunits = int(units.size());
-  p->flops = flops;
-  const int occ = (bt == 128) ? 2 : 4;
-  p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * occ));
-  if (p->nunits > 0) {
-    size_t bt_bytes = st.size() * sizeof(GemmTask);
-    size_t bu_bytes = units.size() * sizeof(WorkUnit);
-    size_t o_units = (bt_bytes + 255) & ~size_t(255);
-    size_t o_tab = (o_units + bu_bytes + 255) & ~size_t(255);
-    size_t total = o_tab + tab.size() * sizeof(int64_t);
-    std::vector<char> host(total, 0);
-    std::memcpy(host.data(), st.data(), bt_bytes);
-    std::memcpy(host.data() + o_units, units.data(), bu_bytes);
-    if (!tab.empty()) std::memcpy(host.data() + o_tab, tab.data(), tab.size() * sizeof(int64_t));
-    hipError_t e = hipMalloc(&p->dev, total);
-    if (e != hipSuccess) {
-      delete p;
-      return hip_err(ctx, e, "hipMalloc(plan)");
-    }
-    e = hipMemcpyAsync(p->dev, host.data(), total, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-      hipFree(p->dev);
-      delete p;
-      return hip_err(ctx, e, "upload(plan)");
-    }
-    p->dev_bytes = total;
-    p->d_tasks = reinterpret_cast<const GemmTask *>(p->dev);
-    p->d_units = reinterpret_cast<const WorkUnit *>(static_cast<char *>(p->dev) + o_units);
-    p->d_tab = reinterpret_cast<const int64_t *>(static_cast<char *>(p->dev) + o_tab);
+  const int cus = std::max(1, ctx->num_cus);
+  int maxN = 0, maxK = 0, maxM = 0;
+  double maxKsum = 0;
+  for (auto &t : st) {
+    maxN = std::max(maxN, t.N);
+    maxK = std::max(maxK, t.K);
+    maxM = std::max(maxM, t.M);
   }
-  *out = p;
-  return ITX_OK;
+  for (auto &g : groups) maxKsum = std::max(maxKsum, g.ksum);
+
+  // Build the device plan for one tile choice.
+  auto finalize = [&](int best, itx_plan **pout) -> int {
+    *pout = nullptr;
+    const int tbm = kTiles[best].bm, tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
+    std::vector<WorkUnit> units;
+    std::vector<double> ucost;
+    for (auto &g : groups) {
+      for (int m0 = 0; m0 < g.M; m0 += tbm)
+        for (int n0 = 0; n0 < g.N; n0 += tbn) {
+          units.push_back({g.tb, g.te, m0, n0});
+          ucost.push_back(g.ksum);
+        }
+    }
+    // heaviest units first (the persistent loop then balances like LPT);
+    // stable so equal-cost tiles keep their m-major order
+    {
+      std::vector<int> ui(units.size());
+      std::iota(ui.begin(), ui.end(), 0);
+      std::stable_sort(ui.begin(), ui.end(), [&](int x, int y) { return ucost[x] > ucost[y]; });
+      std::vector<WorkUnit> su(units.size());
+      for (size_t i = 0; i < ui.size(); ++i) su[i] = units[ui[i]];
+      units.swap(su);
+    }
+    auto *p = new itx_plan();
+    p->prec = prec;
+    p->tile = best;
+    p->a_kcontig = akc;
+    p->b_kcontig = bkc;
+    p->ntasks = int(st.size());
+    p->nunits = int(units.size());
+    p->flops = flops;
+    p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
+    if (std::getenv("ITX_PLAN_LOG"))
+      std::fprintf(stderr, "[itx plan] tasks=%d units=%d tile=%dx%d%s grid=%d akc=%d bkc=%d flops=%.3e\n",
+                   p->ntasks, p->nunits, tbm, tbn, best == kSkinny ? "(skinny)" : "", p->grid, int(akc), int(bkc),
+                   flops);
+    if (p->nunits > 0) {
+      size_t bt_bytes = st.size() * sizeof(GemmTask);
+      size_t bu_bytes = units.size() * sizeof(WorkUnit);
+      size_t o_units = (bt_bytes + 255) & ~size_t(255);
+      size_t o_tab = (o_units + bu_bytes + 255) & ~size_t(255);
+      size_t total = o_tab + tab.size() * sizeof(int64_t);
+      std::vector<char> host(total, 0);
+      std::memcpy(host.data(), st.data(), bt_bytes);
+      std::memcpy(host.data() + o_units, units.data(), bu_bytes);
+      if (!tab.empty()) std::memcpy(host.data() + o_tab, tab.data(), tab.size() * sizeof(int64_t));
+      hipError_t e = hipMalloc(&p->dev, total);
+      if (e != hipSuccess) {
+        delete p;
+        return hip_err(ctx, e, "hipMalloc(plan)");
+      }
+      e = hipMemcpyAsync(p->dev, host.data(), total, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) {
+        (void)hipFree(p->dev);
+        delete p;
+        return hip_err(ctx, e, "upload(plan)");
+      }
+      p->dev_bytes = total;
+      p->d_tasks = reinterpret_cast<const GemmTask *>(p->dev);
+      p->d_units = reinterpret_cast<const WorkUnit *>(static_cast<char *>(p->dev) + o_units);
+      p->d_tab = reinterpret_cast<const int64_t *>(static_cast<char *>(p->dev) + o_tab);
+    }
+    *pout = p;
+    return ITX_OK;
+  };
+
+  // ---- tile selection ------------------------------------------------
+  // Skinny tasks (all N <= 32, K <= 64) take the VALU path.  Otherwise, in
+  // order: ITX_FORCE_TILE, the tuning table entry of this size class, a
+  // measurement of every MFMA tile when autotuning is on (result stored in
+  // the table), else the cost model: units (one per C tile, cost ~ tile
+  // area x summed K) scheduled LPT on CUs x occupancy slots.
+  int best = 0;
+  const std::string key = tune_key(prec, akc, bkc, maxM, maxN, maxKsum, int(groups.size()));
+  const char *force = std::getenv("ITX_FORCE_TILE");
+  if (!st.empty() && maxN <= SK_N && maxK <= SK_K) {
+    best = kSkinny;
+  } else if (force) {
+    best = std::atoi(force);
+    if (best < 0 || best >= kNumTiles || best == kSkinny) best = 0;
+  } else if (ctx->tune.count(key)) {
+    best = ctx->tune[key];
+  } else if (ctx->autotune && !st.empty()) {
+    void *bufA = nullptr, *bufB = nullptr, *bufC = nullptr;
+    int rc2 = pool_alloc(ctx, sizeof(double) * size_t(extA), &bufA);
+    if (!rc2) rc2 = pool_alloc(ctx, sizeof(double) * size_t(extB), &bufB);
+    if (!rc2) rc2 = pool_alloc(ctx, sizeof(double) * size_t(extC), &bufC);
+    if (rc2) return rc2;
+    itx_fill(ctx, static_cast<double *>(bufA), size_t(extA), 0.0);
+    itx_fill(ctx, static_cast<double *>(bufB), size_t(extB), 0.0);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best_ms = 1e30f;
+    for (int ti = 0; ti < kNumTiles; ++ti) {
+      if (ti == kSkinny) continue;
+      itx_plan *pc = nullptr;
+      if (finalize(ti, &pc)) continue;
+      const double *A = static_cast<const double *>(bufA), *B = static_cast<const double *>(bufB);
+      double *Cp = static_cast<double *>(bufC);
+      launch_plan(ctx, pc, A, B, Cp, 1.0, 0.0);
+      (void)hipEventRecord(e0, ctx->stream);
+      for (int r = 0; r < 3; ++r) launch_plan(ctx, pc, A, B, Cp, 1.0, 0.0);
+      (void)hipEventRecord(e1, ctx->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = ti;
+      }
+      itx_plan_destroy(ctx, pc);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    pool_free(ctx, bufA);
+    pool_free(ctx, bufB);
+    pool_free(ctx, bufC);
+    ctx->tune[key] = best;
+  } else {
+    double best_t = 1e300;
+    for (int ti = 0; ti < kNumTiles; ++ti) {
+      if (ti == kSkinny) continue;
+      const TileCfg ts = kTiles[ti];
+      std::vector<double> costs;
+      for (auto &g : groups) {
+        const int64_t tiles = int64_t((g.M + ts.bm - 1) / ts.bm) * ((g.N + ts.bn - 1) / ts.bn);
+        const double c = double(ts.bm) * ts.bn * (g.ksum + BK * (g.te - g.tb) + 8);
+        for (int64_t q = 0; q < tiles; ++q) costs.push_back(c);
+      }
+      if (costs.empty()) continue;
+      const int occ = tile_occupancy(ti, prec, akc, bkc);
+      const double eff = (prec == ITX_PREC_F64) ? ts.eff64 : ts.eff32;
+      const double tt = lpt_makespan(costs, int64_t(cus) * occ) * occ / eff;
+      if (tt < best_t * 0.97) {
+        best_t = tt;
+        best = ti;
+      }
+    }
+  }
+  return finalize(best, out);
 }
 
 int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *B, double *C, double alpha,
                 double beta) {
   if (!p) return set_err(ctx, ITX_ERR_ARG, "null plan");
   if (p->nunits == 0) return ITX_OK;
-  if (p->prec == ITX_PREC_F64)
+  if (p->tile == kSkinny)  // HBM-bound: FP64 arithmetic in every form
+    hipLaunchKernelGGL(k_skinny, dim3(p->grid), dim3(SK_ROWS), 0, ctx->stream, A, B, C, p->d_tasks, p->d_units,
+                       p->nunits, p->d_tab, alpha, beta);
+  else if (p->prec == ITX_PREC_F64)
     launch_prec<double>(p, ctx->stream, A, B, C, alpha, beta);
   else
     launch_prec<float>(p, ctx->stream, A, B, C, alpha, beta);
@@ -716,5 +977,38 @@ int itx_contract_qn(itx_ctx *ctx, const double *A, int rA, const int32_t *labA, 
   if (rc) return rc;
   return itx::launch_plan(ctx, p, A, B, C, alpha, 0.0);
 }
+
+}  // extern "C"
+
+extern "C" {
+
+int itx_set_autotune(itx_ctx *ctx, int on) {
+  if (!ctx) return ITX_ERR_ARG;
+  ctx->autotune = on ? 1 : 0;
+  return ITX_OK;
+}
+
+int itx_tune_save(itx_ctx *ctx, const char *path) {
+  if (!ctx || !path) return ITX_ERR_ARG;
+  std::ofstream f(path);
+  if (!f) return itx::set_err(ctx, ITX_ERR_ARG, std::string("itx_tune_save: cannot write ") + path);
+  for (auto &kv : ctx->tune) f << kv.first << '|' << kv.second << '\n';
+  return ITX_OK;
+}
+
+int itx_tune_load(itx_ctx *ctx, const char *path) {
+  if (!ctx || !path) return ITX_ERR_ARG;
+  std::ifstream f(path);
+  if (!f) return itx::set_err(ctx, ITX_ERR_ARG, std::string("itx_tune_load: cannot read ") + path);
+  std::string line;
+  while (std::getline(f, line)) {
+    auto bar = line.find('|');
+    if (bar == std::string::npos) continue;
+    ctx->tune[line.substr(0, bar)] = std::atoi(line.c_str() + bar + 1);
+  }
+  return ITX_OK;
+}
+
+int itx_tune_size(itx_ctx *ctx) { return ctx ? int(ctx->tune.size()) : 0; }
 
 }  // extern "C"

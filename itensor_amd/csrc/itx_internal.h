@@ -60,6 +60,9 @@ struct itx_ctx {
   // plan cache for the one-shot contraction entry points
   std::unordered_map<std::string, itx_plan *> plan_cache;
   std::vector<std::string> plan_lru;
+  // tile choice per contraction size class (autotuner results)
+  int autotune = 0;
+  std::map<std::string, int> tune;
 };
 
 namespace itx {

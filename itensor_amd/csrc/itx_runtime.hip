@@ -1,5 +1,6 @@
 // Context, stream and caching device allocator of libitx.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -111,6 +112,10 @@ int itx_create(int device, itx_ctx **out) {
     delete ctx;
     return ITX_ERR_NOMEM;
   }
+  // ITX_AUTOTUNE=1 turns the tile autotuner on; ITX_TUNE_FILE names a
+  // tuning table to load (see itx_tune_load)
+  if (const char *a = std::getenv("ITX_AUTOTUNE")) ctx->autotune = std::atoi(a) ? 1 : 0;
+  if (const char *f = std::getenv("ITX_TUNE_FILE")) itx_tune_load(ctx, f);
   *out = ctx;
   return ITX_OK;
 }

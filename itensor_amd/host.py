@@ -43,5 +43,6 @@ dmrg = _h.dmrg
 set_stream = _h.set_stream
 synchronize = _h.synchronize
 set_contract_precision = _h.set_contract_precision
+set_autotune = _h.set_autotune
 
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2

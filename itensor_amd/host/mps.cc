@@ -80,6 +80,50 @@ long LocalOp::size() const {
   return s;
 }
 
+namespace {
+// result index order of A*B (computeLabels + contractIS), no data
+IndexSet productInds(IndexSet const &A, IndexSet const &B) {
+  IndexSet N;
+  std::vector<bool> bc(B.size(), false);
+  std::vector<bool> ac(A.size(), false);
+  for (size_t i = 0; i < A.size(); ++i)
+    for (size_t j = 0; j < B.size(); ++j)
+      if (!bc[j] && A[i] == B[j]) {
+        ac[i] = bc[j] = true;
+        break;
+      }
+  for (size_t i = 0; i < A.size(); ++i)
+    if (!ac[i]) N.push_back(A[i]);
+  for (size_t j = 0; j < B.size(); ++j)
+    if (!bc[j]) N.push_back(B[j]);
+  return N;
+}
+bool sameOrder(IndexSet const &a, IndexSet const &b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (!(a[i] == b[i])) return false;
+  return true;
+}
+}  // namespace
+
+bool LocalOp::setFused(ITensor const *W12, IndexSet const &phiInds) {
+  W12_ = nullptr;
+  if (!W12 || LIsNull() || RIsNull()) return false;
+  IndexSet t1 = productInds(phiInds, L_->inds());
+  IndexSet ref = productInds(productInds(productInds(t1, Op1_->inds()), Op2_->inds()), R_->inds());
+  IndexSet fus = productInds(productInds(t1, W12->inds()), R_->inds());
+  if (!sameOrder(ref, fus)) return false;
+  W12_ = W12;
+  return true;
+}
+
+void LocalMPO::fuseOps(IndexSet const &phiInds) {
+  if (!fuse_ || !lop_.isBulk()) return;
+  auto const &H = *H_;
+  W12_ = H[size_t(b_ - 1)] * H[size_t(b_)];
+  if (!lop_.setFused(&W12_, phiInds)) W12_ = ITensor();
+}
+
 void LocalMPO::position(int b, MPS const &psi) {
   auto t0 = std::chrono::steady_clock::now();
   auto const &H = *H_;
@@ -102,6 +146,7 @@ void LocalMPO::position(int b, MPS const &psi) {
   LHlim_ = b - 1;
   RHlim_ = b + 2;
   lop_.update(&H[size_t(b - 1)], &H[size_t(b)], &PH_[size_t(b - 1)], &PH_[size_t(b + 2)]);
+  b_ = b;
   env_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -142,6 +187,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   if (int(H.size()) != N) throw ITError("dmrg: MPO and MPS lengths differ");
   LocalMPO PH(H);
   PH.setPrecision(int(args.getInt("HpsiPrecision", ITX_PREC_F64)));
+  PH.setFusion(args.getBool("FuseMPO", true));
   Real energy = NAN;
   for (size_t sw = 0; sw < sweeps.size(); ++sw) {
     auto const &S = sweeps[sw];
@@ -157,6 +203,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
       auto t0 = std::chrono::steady_clock::now();
       PH.position(b, psi);
       ITensor phi = psi(b) * psi(b + 1);
+      PH.fuseOps(phi.inds());
       DavidsonStats st;
       energy = davidson(PH, phi, args, &st);
       auto spec = svdBond(psi, b, phi, ha == 1 ? Fromleft : Fromright, args);

@@ -45,13 +45,24 @@ class LocalOp {
     Op2_ = Op2;
     L_ = L;
     R_ = R;
+    W12_ = nullptr;
   }
   bool LIsNull() const { return L_ == nullptr || !(*L_); }
+  bool isBulk() const { return !LIsNull() && !RIsNull(); }
   bool RIsNull() const { return R_ == nullptr || !(*R_); }
   // product (localop.h:324-365), two centre sites; contractions run in the
   // precision form `prec` (ITX_PREC_*), everything else stays FP64
   void product(ITensor const &phi, ITensor &phip) const {
     PrecisionScope ps(prec_);
+    if (W12_ && !LIsNull() && !RIsNull()) {
+      // bulk case with the two MPO tensors pre-contracted (same result
+      // index order as the four-stage chain, checked in setFused)
+      phip = phi * (*L_);
+      phip *= *W12_;
+      phip *= *R_;
+      phip.noPrime();
+      return;
+    }
     if (LIsNull()) {
       phip = phi;
       if (!RIsNull()) phip *= *R_;
@@ -67,9 +78,13 @@ class LocalOp {
   }
   long size() const;  // localop.h:488-530
   void setPrecision(int p) { prec_ = p; }
+  // Use W12 = Op1*Op2 in the bulk case when ((phi*L)*W12)*R yields exactly
+  // the index order of ((phi*L)*Op1)*Op2*R; returns whether it does.
+  bool setFused(ITensor const *W12, IndexSet const &phiInds);
 
  private:
   ITensor const *Op1_ = nullptr, *Op2_ = nullptr, *L_ = nullptr, *R_ = nullptr;
+  ITensor const *W12_ = nullptr;
   int prec_ = ITX_PREC_F64;
 };
 
@@ -84,12 +99,18 @@ class LocalMPO {
   ITensor const &R() const { return PH_.at(size_t(RHlim_)); }
   void setPrecision(int p) { lop_.setPrecision(p); }
   double envSeconds() const { return env_seconds_; }
+  // pre-contract the two MPO tensors of the current bond for phi's index set
+  void fuseOps(IndexSet const &phiInds);
+  void setFusion(bool on) { fuse_ = on; }
 
  private:
   MPO const *H_;
   std::vector<ITensor> PH_;
   int LHlim_, RHlim_;
   LocalOp lop_;
+  ITensor W12_;
+  int b_ = 0;
+  bool fuse_ = true;
   double env_seconds_ = 0;
 };
 

@@ -180,16 +180,18 @@ PYBIND11_MODULE(_itensor_host, m) {
     return py::make_tuple(e, phi, st.products);
   }, "davidson on the two-site problem of bond b (iterativesolvers_test.cc:43-65 pattern)");
   m.def("local_product", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, int b, ITensor const &phi,
-                            int prec) {
+                            int prec, bool fuse) {
     MPO mpo(H);
     MPS psi(A);
     LocalMPO PH(mpo);
     PH.position(b, psi);
     PH.setPrecision(prec);
+    PH.setFusion(fuse);
+    PH.fuseOps(phi.inds());
     ITensor out;
     PH.product(phi, out);
     return out;
-  }, py::arg("H"), py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("prec") = 0);
+  }, py::arg("H"), py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("prec") = 0, py::arg("fuse") = true);
   m.def("dmrg", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, py::list sweeps, py::dict args) {
     MPS psi(A);
     std::vector<py::dict> recs;
@@ -201,4 +203,5 @@ PYBIND11_MODULE(_itensor_host, m) {
   m.def("set_stream", [](uintptr_t s) { setStream(reinterpret_cast<void *>(s)); });
   m.def("synchronize", []() { check(itx_synchronize(ctx()), "itx_synchronize"); });
   m.def("set_contract_precision", &setContractPrecision);
+  m.def("set_autotune", [](bool on) { check(itx_set_autotune(ctx(), on ? 1 : 0), "itx_set_autotune"); });
 }

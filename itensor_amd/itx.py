@@ -67,6 +67,10 @@ _PROTOS = {
     "itx_syev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "itx_syev_batched": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, _i64p, _vp]),
     "itx_block_copy": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _i64p, C.c_double, C.c_double]),
+    "itx_set_autotune": (C.c_int, [_vp, C.c_int]),
+    "itx_tune_save": (C.c_int, [_vp, C.c_char_p]),
+    "itx_tune_load": (C.c_int, [_vp, C.c_char_p]),
+    "itx_tune_size": (C.c_int, [_vp]),
 }
 
 
@@ -182,6 +186,19 @@ class Context:
                                 blkB.ctypes.data_as(_i64p), PREC[prec], C.byref(h))
         self.check(rc, "itx_plan_qn")
         return Plan(self, h)
+
+    # ---- tile autotuner ----
+    def set_autotune(self, on: bool = True):
+        self.check(self.L.itx_set_autotune(self.h, 1 if on else 0), "itx_set_autotune")
+
+    def tune_save(self, path: str):
+        self.check(self.L.itx_tune_save(self.h, path.encode()), "itx_tune_save")
+
+    def tune_load(self, path: str):
+        self.check(self.L.itx_tune_load(self.h, path.encode()), "itx_tune_load")
+
+    def tune_size(self) -> int:
+        return self.L.itx_tune_size(self.h)
 
     # ---- vector kernels ----
     def dot(self, x: int, y: int, n: int) -> float:

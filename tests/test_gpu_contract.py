@@ -136,3 +136,27 @@ def test_bad_labels_raise(ctx, dev):
         ctx.contract_dense(A.data_ptr(), (2, 3), (1, -1), A.data_ptr(), (2, 3), (-2, 2), A.data_ptr(), (2, 3), (1, 2))
     with pytest.raises(ItxError):  # contracted dims differ
         ctx.contract_dense(A.data_ptr(), (2, 3), (1, -1), A.data_ptr(), (2, 3), (-1, 2), A.data_ptr(), (2, 3), (1, 2))
+
+
+def test_autotune_picks_a_tile_and_persists(ctx, dev, tmp_path):
+    rng = np.random.default_rng(21)
+    la, lb, lc = HPSI_STAGES[0]
+    chi = 96
+    psi = rng.standard_normal((chi, 2, 2, chi))
+    L = rng.standard_normal((chi, 5, chi))
+    ref = contract(psi, la, L, lb, lc)
+    ctx.set_autotune(True)
+    try:
+        n0 = ctx.tune_size()
+        plan = ctx.plan_dense(psi.shape, la, L.shape, lb, ref.shape, lc)
+        assert ctx.tune_size() == n0 + 1
+        dA, dB, dC = dev.put(psi), dev.put(L), dev.empty(ref.size)
+        plan.execute(dA.data_ptr(), dB.data_ptr(), dC.data_ptr())
+        assert relerr(dev.get(dC, ref.shape), ref) < TOL64
+        plan.close()
+        path = str(tmp_path / "tune.txt")
+        ctx.tune_save(path)
+        ctx.tune_load(path)
+        assert ctx.tune_size() == n0 + 1
+    finally:
+        ctx.set_autotune(False)

@@ -170,7 +170,10 @@ def test_davidson_closed_form_gpu(H, qn):
     assert abs(e - (-(0.25 + 1 / math.sqrt(2)))) < 1e-10
 
 
-def test_local_product_matches_oracle_dense(H):
+@pytest.mark.parametrize("fuse", [False, True])
+def test_local_product_matches_oracle_dense(H, fuse):
+    """LocalOp::product (four-stage chain, or with the MPO pair fused) vs the
+    oracle's four-stage chain: same result index order, values 1e-12."""
     sites = OD.spin_half_sites(8, False)
     Hm = OD.heisenberg_mpo(sites)
     psi = OD.random_mps(sites, 16, seed=11)
@@ -179,9 +182,28 @@ def test_local_product_matches_oracle_dense(H):
         PH.position(b, psi)
         phi = psi(b) * psi(b + 1)
         ref = PH.product(phi)
-        got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi))
+        got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi), 0, fuse)
         assert same_inds(got, cten(ref))
         assert _rel(got.store(), ref.d.ravel(order="F")) < 1e-12
+
+
+def test_local_product_qn_matches_oracle(H):
+    """QN LocalOp::product: block list bit-exact, values 1e-12."""
+    N = 10
+    sites = OD.spin_half_sites(N, True)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)])
+    _, psi = OD.dmrg(Hm, psi0, [{"MaxDim": 20, "Cutoff": 1e-12}] * 2)
+    PH = OD.LocalMPO(Hm)
+    for b in (1, 5, 9):
+        PH.position(b, psi)
+        phi = psi(b) * psi(b + 1)
+        ref = PH.product(phi)
+        for fuse in (False, True):
+            got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi), 0, fuse)
+            assert same_inds(got, cten(ref))
+            assert [(tuple(x), o) for x, o in got.offsets] == [(x, o) for x, o in ref.offsets]
+            assert _rel(got.store(), ref.store) < 1e-12
 
 
 def test_dmrg_config1_trajectory_matches_oracle(H):
