@@ -110,6 +110,22 @@ def cpu_baseline(psi, L, W, R, budget_s: float = 20.0):
     return ref, float(np.median(times)), threads, reps
 
 
+def copy_calibration(gib: float):
+    """A plain device-to-device copy of known size (reads and writes
+    `gib` GiB per launch), the reference point for HBM byte counters."""
+    import torch
+
+    import itensor_amd  # noqa: F401
+
+    n = int(gib * (1 << 30) / 8)
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n * 8, "bytes_written": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,7 +136,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chain", default="fused", choices=["fused", "4stage"],
                     help="LocalOp::product as run by the host layer (MPO pair fused) or the 4-stage chain")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_gib > 0:
+        return copy_calibration(args.copy_gib)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
                           "n_gpus_requested": args.gpus}))
