@@ -110,6 +110,86 @@ def cpu_baseline(psi, L, W, R, budget_s: float = 20.0):
     return ref, float(np.median(times)), threads, reps
 
 
+# bond sectors q = 2 Sz in {-30..30} of the chi = 1024 Sz-conserving
+# Heisenberg bond at the chain midpoint (binomial weights, max 120, min 1)
+SECTORS_1024 = [1, 1, 1, 1, 1, 1, 3, 7, 14, 25, 41, 60, 81, 101, 114, 120, 114, 101, 81, 60, 41, 25, 14, 7, 3,
+                1, 1, 1, 1, 1, 1]
+
+
+def qn_operands(H, sectors, seed):
+    """Block-sparse two-site operands: phi(dag(l), s1, s2, r), L(l, hl,
+    dag(l)'), R(dag(r), dag(hr), r'), W(dag(s), s', dag(h_in), h_out);
+    site {+1: 1, -1: 1}, MPO link {0: 3, -2: 1, +2: 1}; flux-0 blocks of
+    phi, L, R filled with N(0,1)."""
+    def sz(v):
+        return [("Sz", int(v))]
+
+    ids = iter(range(7001, 7100))
+
+    def idx(qns, tags):
+        return H.Index(next(ids), sum(d for _, d in qns), 1, 0, tags, qns)
+
+    bond = [(sz(q), d) for q, d in zip(range(-30, 31, 2), sectors)]
+    l, r = idx(bond, "Link,l=49"), idx(bond, "Link,l=51")
+    site = [(sz(1), 1), (sz(-1), 1)]
+    s1, s2 = idx(site, "Site,n=50"), idx(site, "Site,n=51")
+    hq = [(sz(0), 3), (sz(-2), 1), (sz(2), 1)]
+    hl, hm, hr = idx(hq, "Link,h=49"), idx(hq, "Link,h=50"), idx(hq, "Link,h=51")
+    W = heisenberg_w().transpose(1, 2, 0, 3)  # (s, s', h_in, h_out)
+    W1 = H.from_dense([s1.dag(), s1.prime(), hl.dag(), hm], W)
+    W2 = H.from_dense([s2.dag(), s2.prime(), hm.dag(), hr], W)
+    phi = H.random_itensor([l.dag(), s1, s2, r], [], seed)
+    L = H.random_itensor([l, hl, l.dag().prime()], [], seed + 1)
+    R = H.random_itensor([r.dag(), hr.dag(), r.prime()], [], seed + 2)
+    return phi, L, W1, W2, R
+
+
+def qn_hpsi(steps: int, warmup: int):
+    """Block-sparse H*psi (chi = 1024 over 31 Sz sectors) through
+    LocalOp::product in each execution mode: the grouped-GEMM chain
+    replayed as a HIP graph, the same chain launched directly, eager
+    operator* products, and one launch per block pair (the reference's
+    loop).  Wall time per product over `steps` products after `warmup`."""
+    import torch
+
+    from itensor_amd import host as H
+
+    phi, L, W1, W2, R = qn_operands(H, SECTORS_1024, 20261202 + 3)
+    modes = [("graph", H.HPSI_GRAPH, 0), ("planned", H.HPSI_PLANNED, 0), ("eager", H.HPSI_EAGER, 0),
+             ("per_block", H.HPSI_PER_BLOCK, 0), ("graph_f32", H.HPSI_GRAPH, 1)]
+    res, ref = {}, None
+    for name, mode, prec in modes:
+        op = H.LocalOp(W1, W2, L, R, True)
+        op.set_mode(mode)
+        op.set_precision(prec)
+        flops = op.flops(phi) if mode != H.HPSI_EAGER else res["graph"]["flops"]
+        t_plan = time.perf_counter()
+        out = op.product(phi)
+        torch.cuda.synchronize()
+        t_plan = time.perf_counter() - t_plan
+        for _ in range(warmup):
+            op.product(phi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            op.product(phi)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        st = out.store()
+        r = {"ms_per_hpsi": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 3), "flops": flops,
+             "first_call_ms": round(t_plan * 1e3, 2)}
+        if ref is None:
+            ref = (st, out.offsets)
+            r["blocks_out"] = len(out.offsets)
+        else:
+            r["same_block_list"] = out.offsets == ref[1]
+            r["rel_err_vs_graph_f64"] = float(np.max(np.abs(st - ref[0])) / np.max(np.abs(ref[0])))
+        res[name] = r
+    res["blocks_phi"] = len(phi.offsets)
+    res["speedup_grouped_vs_per_block"] = round(res["per_block"]["ms_per_hpsi"] / res["graph"]["ms_per_hpsi"], 2)
+    return res
+
+
 def copy_calibration(gib: float):
     """A plain device-to-device copy of known size (reads and writes
     `gib` GiB per launch), the reference point for HBM byte counters."""
@@ -136,6 +216,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chain", default="fused", choices=["fused", "4stage"],
                     help="LocalOp::product as run by the host layer (MPO pair fused) or the 4-stage chain")
+    ap.add_argument("--no-qn", action="store_true", help="skip the block-sparse (31 Sz sectors) H*psi leg")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
@@ -254,6 +335,11 @@ def main():
         line["stages_ms"][prec] = [round(x, 4) for x in r["stage_ms"]]
         line["forms"][prec] = {"tflops": round(flops / (r["wall_s"] / args.steps) / 1e12, 3),
                                "ms_per_hpsi": round(r["wall_s"] / args.steps * 1e3, 4)}
+
+    if not args.no_qn:
+        q = qn_hpsi(max(5, args.steps), args.warmup)
+        line["qn_hpsi"] = {"workload": "block-sparse two-site Hpsi, chi=1024 over 31 Sz sectors "
+                                       "(max 120, min 1), site {+1,-1}, MPO link {0:3,-2:1,+2:1}", **q}
 
     if not args.no_cpu:
         ref, t_cpu, threads, reps = cpu_baseline(psi, L, W, R)

@@ -57,6 +57,7 @@ extern "C" {
 
 typedef struct itx_ctx itx_ctx;
 typedef struct itx_plan itx_plan;
+typedef struct itx_chain itx_chain;
 
 /* ---- context, stream, memory ---------------------------------------- */
 /* Create a context on HIP device `device` using the null stream. */
@@ -146,6 +147,21 @@ int itx_tune_size(itx_ctx *ctx);
 /* Algorithmic FP64-equivalent flops of one execution (sum of 2*m*n*k). */
 double itx_plan_flops(const itx_plan *plan);
 int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan);
+
+/* ---- H*psi chains replayed as a HIP graph -------------------------------
+ * A chain is a fixed sequence of plan executions, stage i computing
+ * C[i] = alpha[i] * contract(A[i], B[i]) + beta[i] * C[i] (the stages of
+ * LocalOp::product, mps/localop.h:346-364, whose structure stays fixed
+ * across the products of one Davidson call, iterativesolvers.h:160-330).
+ * The first itx_chain_launch builds one hipGraph with a kernel node per
+ * stage (each after the previous one) and instantiates it; later launches
+ * patch the operand pointers and scalars of the nodes whose values changed
+ * and replay the graph on the context's stream.  alpha / beta may be NULL
+ * (1 and 0).  The chain does not own the plans: they must outlive it. */
+int itx_chain_create(itx_ctx *ctx, int nstages, itx_plan *const *plans, itx_chain **out);
+int itx_chain_launch(itx_ctx *ctx, itx_chain *chain, const double *const *A, const double *const *B,
+                     double *const *C, const double *alpha, const double *beta);
+int itx_chain_destroy(itx_ctx *ctx, itx_chain *chain);
 
 /* ---- vector kernels (FP64, wavefront reductions) ----------------------
  * Results of reductions are written to host memory (the call waits for

@@ -464,88 +464,58 @@ constexpr int kNumTiles = 10;
 constexpr int kSkinny = 5;
 constexpr int kSkinnyOcc = 4;
 
-template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC>
-void launch_t(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C, double alpha,
-              double beta) {
-  hipLaunchKernelGGL((k_gemm<T, BM, BN, GM, GN, AKC, BKC>), dim3(p->grid), dim3(64 * GM * GN), 0, s, A, B, C,
-                     p->d_tasks, p->d_units, p->nunits, p->d_tab, alpha, beta);
-}
-
+// Kernel instantiation of a tile shape, operand layout and form.  All
+// kernels of the engine share one argument list (see KernelArgs).
 template <typename T, int BM, int BN, int GM, int GN>
-void launch_bm(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C, double alpha,
-               double beta) {
-  if (p->a_kcontig) {
-    if (p->b_kcontig)
-      launch_t<T, BM, BN, GM, GN, true, true>(p, s, A, B, C, alpha, beta);
-    else
-      launch_t<T, BM, BN, GM, GN, true, false>(p, s, A, B, C, alpha, beta);
-  } else {
-    if (p->b_kcontig)
-      launch_t<T, BM, BN, GM, GN, false, true>(p, s, A, B, C, alpha, beta);
-    else
-      launch_t<T, BM, BN, GM, GN, false, false>(p, s, A, B, C, alpha, beta);
-  }
+const void *gemm_fn(bool akc, bool bkc) {
+  if (akc)
+    return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true>)
+               : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false>);
+  return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true>)
+             : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false>);
 }
 
 template <typename T>
-void launch_prec(const itx_plan *p, hipStream_t s, const double *A, const double *B, double *C,
-                 double alpha, double beta) {
-  switch (p->tile) {
-    case 0: launch_bm<T, 128, 128, 2, 2>(p, s, A, B, C, alpha, beta); break;
-    case 1: launch_bm<T, 128, 64, 2, 2>(p, s, A, B, C, alpha, beta); break;
-    case 2: launch_bm<T, 64, 128, 2, 2>(p, s, A, B, C, alpha, beta); break;
-    case 3: launch_bm<T, 64, 64, 2, 2>(p, s, A, B, C, alpha, beta); break;
-    case 6: launch_bm<T, 128, 128, 2, 4>(p, s, A, B, C, alpha, beta); break;
-    case 7: launch_bm<T, 256, 64, 4, 2>(p, s, A, B, C, alpha, beta); break;
-    case 8: launch_bm<T, 64, 256, 2, 4>(p, s, A, B, C, alpha, beta); break;
-    case 9: launch_bm<T, 256, 128, 4, 2>(p, s, A, B, C, alpha, beta); break;
-    default: launch_bm<T, 32, 32, 2, 2>(p, s, A, B, C, alpha, beta); break;
+const void *gemm_fn_tile(int tile, bool akc, bool bkc) {
+  switch (tile) {
+    case 0: return gemm_fn<T, 128, 128, 2, 2>(akc, bkc);
+    case 1: return gemm_fn<T, 128, 64, 2, 2>(akc, bkc);
+    case 2: return gemm_fn<T, 64, 128, 2, 2>(akc, bkc);
+    case 3: return gemm_fn<T, 64, 64, 2, 2>(akc, bkc);
+    case 6: return gemm_fn<T, 128, 128, 2, 4>(akc, bkc);
+    case 7: return gemm_fn<T, 256, 64, 4, 2>(akc, bkc);
+    case 8: return gemm_fn<T, 64, 256, 2, 4>(akc, bkc);
+    case 9: return gemm_fn<T, 256, 128, 4, 2>(akc, bkc);
+    default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc);
   }
+}
+
+// Entry point and workgroup size of a plan's kernel.
+const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block) {
+  if (tile == kSkinny) {  // HBM-bound: FP64 arithmetic in every form
+    *block = SK_ROWS;
+    return reinterpret_cast<const void *>(&k_skinny);
+  }
+  *block = 64 * kTiles[tile].gm * kTiles[tile].gn;
+  return prec == ITX_PREC_F64 ? gemm_fn_tile<double>(tile, akc, bkc) : gemm_fn_tile<float>(tile, akc, bkc);
 }
 
 // Resident workgroups per CU of one kernel instantiation (registers and
 // LDS), from the runtime's occupancy calculator; cached per variant.
-template <typename T, int BM, int BN, int GM, int GN>
-int occ_bm(bool akc, bool bkc) {
-  const void *f;
-  if (akc)
-    f = bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true>)
-            : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false>);
-  else
-    f = bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true>)
-            : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false>);
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * GM * GN, 0) != hipSuccess || nb < 1) {
-    (void)hipGetLastError();
-    nb = 1;
-  }
-  return nb;
-}
-
-template <typename T>
-int occ_prec(int tile, bool akc, bool bkc) {
-  switch (tile) {
-    case 0: return occ_bm<T, 128, 128, 2, 2>(akc, bkc);
-    case 1: return occ_bm<T, 128, 64, 2, 2>(akc, bkc);
-    case 2: return occ_bm<T, 64, 128, 2, 2>(akc, bkc);
-    case 3: return occ_bm<T, 64, 64, 2, 2>(akc, bkc);
-    case 6: return occ_bm<T, 128, 128, 2, 4>(akc, bkc);
-    case 7: return occ_bm<T, 256, 64, 4, 2>(akc, bkc);
-    case 8: return occ_bm<T, 64, 256, 2, 4>(akc, bkc);
-    case 9: return occ_bm<T, 256, 128, 4, 2>(akc, bkc);
-    default: return occ_bm<T, 32, 32, 2, 2>(akc, bkc);
-  }
-}
-
 int tile_occupancy(int tile, int prec, bool akc, bool bkc) {
   static std::map<int, int> cache;
   if (tile == kSkinny) return kSkinnyOcc;
   const int key = tile * 16 + prec * 4 + (akc ? 2 : 0) + (bkc ? 1 : 0);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  const int o = (prec == ITX_PREC_F64) ? occ_prec<double>(tile, akc, bkc) : occ_prec<float>(tile, akc, bkc);
-  cache[key] = o;
-  return o;
+  int block = 0, nb = 0;
+  const void *f = kernel_of(tile, prec, akc, bkc, &block);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, 0) != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  cache[key] = nb;
+  return nb;
 }
 
 // Makespan of units (costs) on `slots` identical workers, heaviest first
@@ -745,6 +715,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->nunits = int(units.size());
     p->flops = flops;
     p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
+    p->fn = kernel_of(best, prec, akc, bkc, &p->block);
     if (std::getenv("ITX_PLAN_LOG"))
       std::fprintf(stderr, "[itx plan] tasks=%d units=%d tile=%dx%d%s grid=%d akc=%d bkc=%d flops=%.3e\n",
                    p->ntasks, p->nunits, tbm, tbn, best == kSkinny ? "(skinny)" : "", p->grid, int(akc), int(bkc),
@@ -861,14 +832,10 @@ int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *
                 double beta) {
   if (!p) return set_err(ctx, ITX_ERR_ARG, "null plan");
   if (p->nunits == 0) return ITX_OK;
-  if (p->tile == kSkinny)  // HBM-bound: FP64 arithmetic in every form
-    hipLaunchKernelGGL(k_skinny, dim3(p->grid), dim3(SK_ROWS), 0, ctx->stream, A, B, C, p->d_tasks, p->d_units,
-                       p->nunits, p->d_tab, alpha, beta);
-  else if (p->prec == ITX_PREC_F64)
-    launch_prec<double>(p, ctx->stream, A, B, C, alpha, beta);
-  else
-    launch_prec<float>(p, ctx->stream, A, B, C, alpha, beta);
-  ITX_CHECK_HIP(ctx, hipGetLastError());
+  KernelArgs ka{A, B, C, p->d_tasks, p->d_units, p->nunits, p->d_tab, alpha, beta};
+  void *args[KernelArgs::kCount];
+  ka.pointers(args);
+  ITX_CHECK_HIP(ctx, hipLaunchKernel(p->fn, dim3(p->grid), dim3(p->block), args, 0, ctx->stream));
   return ITX_OK;
 }
 
@@ -961,6 +928,81 @@ int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan) {
     hipFree(plan->dev);
   }
   delete plan;
+  return ITX_OK;
+}
+
+int itx_chain_create(itx_ctx *ctx, int nstages, itx_plan *const *plans, itx_chain **out) {
+  if (!ctx || !out || nstages <= 0 || !plans) return ITX_ERR_ARG;
+  *out = nullptr;
+  for (int i = 0; i < nstages; ++i)
+    if (!plans[i]) return itx::set_err(ctx, ITX_ERR_ARG, "itx_chain_create: null plan");
+  auto *ch = new itx_chain();
+  ch->plans.assign(plans, plans + nstages);
+  ch->args.resize(size_t(nstages));
+  ch->nodes.assign(size_t(nstages), nullptr);
+  *out = ch;
+  return ITX_OK;
+}
+
+int itx_chain_launch(itx_ctx *ctx, itx_chain *ch, const double *const *A, const double *const *B,
+                     double *const *C, const double *alpha, const double *beta) {
+  if (!ctx || !ch || !A || !B || !C) return ITX_ERR_ARG;
+  const int n = int(ch->plans.size());
+  auto args_of = [&](int i) {
+    const itx_plan *p = ch->plans[size_t(i)];
+    return itx::KernelArgs{A[i], B[i], C[i], p->d_tasks, p->d_units, p->nunits, p->d_tab,
+                           alpha ? alpha[i] : 1.0, beta ? beta[i] : 0.0};
+  };
+  auto params_of = [&](int i, void **argv) {
+    const itx_plan *p = ch->plans[size_t(i)];
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void *>(p->fn);
+    kp.gridDim = dim3(unsigned(p->grid));
+    kp.blockDim = dim3(unsigned(p->block));
+    kp.sharedMemBytes = 0;
+    ch->args[size_t(i)].pointers(argv);
+    kp.kernelParams = argv;
+    kp.extra = nullptr;
+    return kp;
+  };
+  if (!ch->exec) {
+    // first launch: one kernel node per non-empty stage, each depending on
+    // the previous one (stream order of the eager path)
+    ITX_CHECK_HIP(ctx, hipGraphCreate(&ch->graph, 0));
+    hipGraphNode_t prev = nullptr;
+    for (int i = 0; i < n; ++i) {
+      ch->args[size_t(i)] = args_of(i);
+      if (ch->plans[size_t(i)]->nunits == 0) continue;
+      void *argv[itx::KernelArgs::kCount];
+      hipKernelNodeParams kp = params_of(i, argv);
+      ITX_CHECK_HIP(ctx, hipGraphAddKernelNode(&ch->nodes[size_t(i)], ch->graph, prev ? &prev : nullptr,
+                                               prev ? 1 : 0, &kp));
+      prev = ch->nodes[size_t(i)];
+    }
+    ITX_CHECK_HIP(ctx, hipGraphInstantiate(&ch->exec, ch->graph, nullptr, nullptr, 0));
+  } else {
+    // replay: patch the nodes whose operands or scalars changed
+    for (int i = 0; i < n; ++i) {
+      if (!ch->nodes[size_t(i)]) continue;
+      const itx::KernelArgs a = args_of(i);
+      itx::KernelArgs &c = ch->args[size_t(i)];
+      if (a.A == c.A && a.B == c.B && a.C == c.C && a.alpha == c.alpha && a.beta == c.beta) continue;
+      c = a;
+      void *argv[itx::KernelArgs::kCount];
+      hipKernelNodeParams kp = params_of(i, argv);
+      ITX_CHECK_HIP(ctx, hipGraphExecKernelNodeSetParams(ch->exec, ch->nodes[size_t(i)], &kp));
+    }
+  }
+  ITX_CHECK_HIP(ctx, hipGraphLaunch(ch->exec, ctx->stream));
+  return ITX_OK;
+}
+
+int itx_chain_destroy(itx_ctx *ctx, itx_chain *ch) {
+  if (!ch) return ITX_OK;
+  if (ctx) hipStreamSynchronize(ctx->stream);
+  if (ch->exec) hipGraphExecDestroy(ch->exec);
+  if (ch->graph) hipGraphDestroy(ch->graph);
+  delete ch;
   return ITX_OK;
 }
 

@@ -30,6 +30,23 @@ struct WorkUnit {
   int32_t tbeg, tend, m0, n0;
 };
 
+// The argument list shared by every contraction kernel (k_gemm variants
+// and k_skinny), in declaration order.
+struct KernelArgs {
+  const double *A, *B;
+  double *C;
+  const GemmTask *tasks;
+  const WorkUnit *units;
+  int nunits;
+  const int64_t *tab;
+  double alpha, beta;
+  static constexpr int kCount = 9;
+  void pointers(void **a) {
+    a[0] = &A; a[1] = &B; a[2] = &C; a[3] = &tasks; a[4] = &units;
+    a[5] = &nunits; a[6] = &tab; a[7] = &alpha; a[8] = &beta;
+  }
+};
+
 }  // namespace itx
 
 struct itx_plan {
@@ -44,6 +61,17 @@ struct itx_plan {
   const itx::GemmTask *d_tasks = nullptr;
   const itx::WorkUnit *d_units = nullptr;
   const int64_t *d_tab = nullptr;
+  const void *fn = nullptr;  // kernel entry point and workgroup size
+  int block = 0;
+};
+
+// A fixed sequence of plan executions replayed as one HIP graph.
+struct itx_chain {
+  std::vector<const itx_plan *> plans;
+  std::vector<itx::KernelArgs> args;  // arguments the graph nodes hold now
+  std::vector<hipGraphNode_t> nodes;  // per stage; null for empty plans
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct itx_ctx {

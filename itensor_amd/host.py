@@ -44,5 +44,10 @@ set_stream = _h.set_stream
 synchronize = _h.synchronize
 set_contract_precision = _h.set_contract_precision
 set_autotune = _h.set_autotune
+LocalOp = _h.LocalOp
+random_itensor = _h.random_itensor
+from_dense = _h.from_dense
+HPSI_EAGER, HPSI_PLANNED, HPSI_GRAPH, HPSI_PER_BLOCK = (_h.HPSI_EAGER, _h.HPSI_PLANNED, _h.HPSI_GRAPH,
+                                                        _h.HPSI_PER_BLOCK)
 
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2

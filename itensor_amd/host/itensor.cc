@@ -126,14 +126,18 @@ ITensor contractDense(ITensor const &A, ITensor const &B) {
   return ITensor(N, st);
 }
 
-// getContractedOffsets (itdata/qutil.h:93-242) + the block loop of
-// doTask(Contract,QDense,QDense) (itdata/qdense.cc:671-743)
-ITensor contractQN(ITensor const &A, ITensor const &B) {
-  std::vector<int32_t> la, lb, lc;
-  computeLabels(A.inds(), B.inds(), la, lb);
-  IndexSet N;
-  contractIS(A.inds(), la, B.inds(), lb, N, lc);
-  IndexSet const &Ais = A.inds(), &Bis = B.inds();
+}  // namespace
+
+// getContractedOffsets (itdata/qutil.h:93-242): labels, result indices, the
+// reachable block pairs (A block x B block with equal contracted sectors,
+// :155-202) and the result block list, sorted in block order and
+// deduplicated, with prefix-sum offsets (:221-239).  Structure only.
+QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet const &Bis,
+                      BlockOffsets const &Boffs) {
+  QNTaskList tl;
+  computeLabels(Ais, Bis, tl.la, tl.lb);
+  contractIS(Ais, tl.la, Bis, tl.lb, tl.N, tl.lc);
+  IndexSet const &N = tl.N;
   const int rA = int(Ais.size()), rB = int(Bis.size()), rC = int(N.size());
   std::vector<int> AtoB(rA, -1), AtoC(rA, -1), BtoC(rB, -1);
   for (int ic = 0; ic < rC; ++ic) {
@@ -151,22 +155,23 @@ ITensor contractQN(ITensor const &A, ITensor const &B) {
       }
   struct Triple {
     int64_t aoff, boff;
-    Block a, b, c;
+    Block const *a, *b;
+    Block c;
   };
   std::vector<Triple> triples;
   BlockOffsets csizes;
   Block cblk(rC, 0);
-  for (auto const &aio : A.store().offsets) {
+  for (auto const &aio : Aoffs) {
     for (int ia = 0; ia < rA; ++ia)
       if (AtoC[ia] != -1) cblk[AtoC[ia]] = aio.block[ia];
-    for (auto const &bio : B.store().offsets) {
+    for (auto const &bio : Boffs) {
       bool ok = true;
       for (int ia = 0; ia < rA && ok; ++ia)
         if (AtoB[ia] != -1 && aio.block[ia] != bio.block[AtoB[ia]]) ok = false;
       if (!ok) continue;
       for (int ib = 0; ib < rB; ++ib)
         if (BtoC[ib] != -1) cblk[BtoC[ib]] = bio.block[ib];
-      triples.push_back({aio.offset, bio.offset, aio.block, bio.block, cblk});
+      triples.push_back({aio.offset, bio.offset, &aio.block, &bio.block, cblk});
       csizes.push_back({cblk, prod(blockDims(N, cblk))});
     }
   }
@@ -175,28 +180,48 @@ ITensor contractQN(ITensor const &A, ITensor const &B) {
                csizes.end());
   int64_t cur = 0;
   for (auto &c : csizes) {
-    int64_t s = c.offset;
+    int64_t sz = c.offset;
     c.offset = cur;
-    cur += s;
+    cur += sz;
   }
-  auto st = mkQDense(csizes, size_t(cur), false);
+  tl.coffs = std::move(csizes);
+  tl.csize = cur;
   const int nt = int(triples.size());
-  if (nt > 0) {
-    std::vector<int64_t> offs(3 * size_t(nt)), bda(size_t(nt) * rA), bdb(size_t(nt) * rB);
-    for (int t = 0; t < nt; ++t) {
-      offs[3 * t] = triples[t].aoff;
-      offs[3 * t + 1] = triples[t].boff;
-      offs[3 * t + 2] = offsetOf(st->offsets, triples[t].c);
-      auto da = blockDims(Ais, triples[t].a);
-      auto db = blockDims(Bis, triples[t].b);
-      std::copy(da.begin(), da.end(), bda.begin() + size_t(t) * rA);
-      std::copy(db.begin(), db.end(), bdb.begin() + size_t(t) * rB);
-    }
-    check(itx_contract_qn(ctx(), A.store().data->p, rA, la.data(), B.store().data->p, rB, lb.data(), st->data->p, rC,
-                          lc.data(), nt, offs.data(), bda.data(), bdb.data(), 1.0, contractPrecision()),
+  tl.nt = nt;
+  tl.offs.resize(3 * size_t(nt));
+  tl.bda.resize(size_t(nt) * rA);
+  tl.bdb.resize(size_t(nt) * rB);
+  tl.cfirst.assign(size_t(nt), 0);
+  std::vector<char> seen(tl.coffs.size(), 0);
+  for (int t = 0; t < nt; ++t) {
+    tl.offs[3 * t] = triples[t].aoff;
+    tl.offs[3 * t + 1] = triples[t].boff;
+    auto it = std::lower_bound(tl.coffs.begin(), tl.coffs.end(), triples[t].c, BlockCmp());
+    tl.offs[3 * t + 2] = it->offset;
+    const size_t ci = size_t(it - tl.coffs.begin());
+    if (!seen[ci]) tl.cfirst[size_t(t)] = seen[ci] = 1;
+    auto da = blockDims(Ais, *triples[t].a);
+    auto db = blockDims(Bis, *triples[t].b);
+    std::copy(da.begin(), da.end(), tl.bda.begin() + size_t(t) * rA);
+    std::copy(db.begin(), db.end(), tl.bdb.begin() + size_t(t) * rB);
+  }
+  return tl;
+}
+
+namespace {
+
+// the block loop of doTask(Contract,QDense,QDense) (itdata/qdense.cc:671-743)
+// as one grouped launch
+ITensor contractQN(ITensor const &A, ITensor const &B) {
+  QNTaskList tl = qnTaskList(A.inds(), A.store().offsets, B.inds(), B.store().offsets);
+  auto st = mkQDense(tl.coffs, size_t(tl.csize), false);
+  if (tl.nt > 0) {
+    check(itx_contract_qn(ctx(), A.store().data->p, A.order(), tl.la.data(), B.store().data->p, B.order(),
+                          tl.lb.data(), st->data->p, int(tl.N.size()), tl.lc.data(), tl.nt, tl.offs.data(),
+                          tl.bda.data(), tl.bdb.data(), 1.0, contractPrecision()),
           "itx_contract_qn");
   }
-  return ITensor(N, st);
+  return ITensor(tl.N, st);
 }
 
 // -------------------------------------------------------------- block copy --
@@ -402,6 +427,86 @@ bool sameOffsets(BlockOffsets const &a, BlockOffsets const &b) {
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ ContractPlan --
+ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool perBlock) : perBlock_(perBlock) {
+  if (!A || !B) throw ITError("ContractPlan: default constructed ITensor");
+  if (A.isCombiner() || B.isCombiner()) throw ITError("ContractPlan: combiner operand");
+  if (A.isQN() != B.isQN()) throw ITError("ContractPlan: QN and dense operands");
+  checkArrows(A.inds(), B.inds());
+  qn_ = A.isQN();
+  if (!qn_) {
+    std::vector<int32_t> la, lb, lc;
+    computeLabels(A.inds(), B.inds(), la, lb);
+    contractIS(A.inds(), la, B.inds(), lb, N_, lc);
+    auto dA = dimsOf(A.inds()), dB = dimsOf(B.inds()), dC = dimsOf(N_);
+    size_ = size_t(prod(dC));
+    check(itx_plan_dense(ctx(), int(dA.size()), dA.data(), la.data(), int(dB.size()), dB.data(), lb.data(),
+                         int(dC.size()), dC.data(), lc.data(), prec, &plan_),
+          "itx_plan_dense");
+    flops_ = itx_plan_flops(plan_);
+    return;
+  }
+  QNTaskList tl = qnTaskList(A.inds(), A.store().offsets, B.inds(), B.store().offsets);
+  N_ = tl.N;
+  coffs_ = tl.coffs;
+  size_ = size_t(tl.csize);
+  const int rA = A.order(), rB = B.order(), rC = int(N_.size());
+  if (tl.nt == 0) return;
+  if (!perBlock) {
+    check(itx_plan_qn(ctx(), rA, tl.la.data(), rB, tl.lb.data(), rC, tl.lc.data(), tl.nt, tl.offs.data(),
+                      tl.bda.data(), tl.bdb.data(), prec, &plan_),
+          "itx_plan_qn");
+    flops_ = itx_plan_flops(plan_);
+    return;
+  }
+  try {
+    for (int t = 0; t < tl.nt; ++t) {
+      itx_plan *p = nullptr;
+      check(itx_plan_qn(ctx(), rA, tl.la.data(), rB, tl.lb.data(), rC, tl.lc.data(), 1, tl.offs.data() + 3 * t,
+                        tl.bda.data() + size_t(t) * rA, tl.bdb.data() + size_t(t) * rB, prec, &p),
+            "itx_plan_qn");
+      blockPlans_.push_back(p);
+      blockBeta_.push_back(tl.cfirst[size_t(t)] ? 0.0 : 1.0);
+      flops_ += itx_plan_flops(p);
+    }
+  } catch (...) {
+    for (auto *p : blockPlans_) itx_plan_destroy(ctx(), p);
+    throw;
+  }
+}
+
+ContractPlan::~ContractPlan() {
+  if (plan_) itx_plan_destroy(ctx(), plan_);
+  for (auto *p : blockPlans_) itx_plan_destroy(ctx(), p);
+}
+
+std::shared_ptr<Storage> ContractPlan::newOutput() const {
+  return qn_ ? mkQDense(coffs_, size_, false) : mkDense(size_);
+}
+
+void ContractPlan::exec(double const *A, double const *B, double *C) const {
+  if (plan_) {
+    check(itx_plan_execute(ctx(), plan_, A, B, C, 1.0, 0.0), "itx_plan_execute");
+    return;
+  }
+  for (size_t t = 0; t < blockPlans_.size(); ++t)
+    check(itx_plan_execute(ctx(), blockPlans_[t], A, B, C, 1.0, blockBeta_[t]), "itx_plan_execute");
+}
+
+bool ContractPlan::sameStructure(ITensor const &a, IndexSet const &is, bool qn, BlockOffsets const &offs) {
+  if (!a || a.isCombiner() || a.isQN() != qn) return false;
+  IndexSet const &x = a.inds();
+  if (x.size() != is.size()) return false;
+  for (size_t i = 0; i < x.size(); ++i)
+    if (!(x[i] == is[i]) || x[i].dim() != is[i].dim() || x[i].dir() != is[i].dir()) return false;
+  if (!qn) return true;
+  BlockOffsets const &o = a.store().offsets;
+  if (o.size() != offs.size()) return false;
+  for (size_t i = 0; i < o.size(); ++i)
+    if (o[i].offset != offs[i].offset || o[i].block != offs[i].block) return false;
+  return true;
+}
 
 // ---------------------------------------------------------------- ITensor --
 void ITensor::detach() {
@@ -809,6 +914,46 @@ void randomize(ITensor &T, uint64_t seed) {
   }
   if (!h.empty())
     check(itx_memcpy_htod(ctx(), T.store().data->p, h.data(), sizeof(double) * h.size()), "itx_memcpy_htod");
+}
+
+// randomITensor(QN, IndexSet) / randomITensor(IndexSet) (itensor.h:969-975):
+// every flux-allowed block (or the dense tensor) filled with N(0,1) from
+// the seeded SplitMix64 + Box-Muller stream
+ITensor randomITensor(QN const &div, IndexSet const &is, uint64_t seed) {
+  ITensor T = hasQNs(is) ? qdenseZeros(is, div) : ITensor(is, mkDense(size_t(dim(is))));
+  randomize(T, seed);
+  return T;
+}
+
+// QDense tensor of flux div holding the entries of a dense column-major
+// array; entries outside the flux-allowed blocks must be zero
+ITensor qdenseFromDense(IndexSet const &is, std::vector<double> const &colmajor, QN const &div) {
+  if (long(colmajor.size()) != dim(is)) throw ITError("qdenseFromDense: size mismatch");
+  auto [offs, n] = getBlockOffsets(is, div);
+  std::vector<double> store(size_t(n), 0.0);
+  std::vector<char> used(colmajor.size(), 0);
+  const size_t r = is.size();
+  auto strides = colStrides(dimsOf(is));
+  for (auto const &bo : offs) {
+    auto bd = blockDims(is, bo.block);
+    std::vector<int64_t> start(r);
+    for (size_t j = 0; j < r; ++j) start[j] = is[j].blockStart(bo.block[j]);
+    const int64_t nb = prod(bd);
+    std::vector<int64_t> i(r, 0);
+    for (int64_t e = 0; e < nb; ++e) {
+      int64_t src = 0;
+      for (size_t j = 0; j < r; ++j) src += (start[j] + i[j]) * strides[j];
+      store[size_t(bo.offset + e)] = colmajor[size_t(src)];
+      used[size_t(src)] = 1;
+      for (size_t j = 0; j < r; ++j) {
+        if (++i[j] < bd[j]) break;
+        i[j] = 0;
+      }
+    }
+  }
+  for (size_t k = 0; k < colmajor.size(); ++k)
+    if (!used[k] && colmajor[k] != 0.0) throw ITError("qdenseFromDense: nonzero entry outside the flux-allowed blocks");
+  return qdenseFromHost(is, offs, store);
 }
 
 }  // namespace itensor

@@ -289,6 +289,64 @@ ITensor qdenseFromHost(IndexSet const &is, BlockOffsets const &offs, std::vector
 ITensor qdenseZeros(IndexSet const &is, QN const &div);
 std::vector<double> storageToHost(ITensor const &T);
 void randomize(ITensor &T, uint64_t seed);
+// randomITensor(QN, IndexSet) (itensor.h:969-975): N(0,1) entries in every
+// flux-allowed block (dense when the indices carry no QNs)
+ITensor randomITensor(QN const &div, IndexSet const &is, uint64_t seed);
+// QDense of flux div from a dense column-major array (zero elsewhere)
+ITensor qdenseFromDense(IndexSet const &is, std::vector<double> const &colmajor, QN const &div);
+
+// ------------------------------------------------- planned contractions --
+// Block-pair task list of a QN contraction, from the operands' structure
+// (getContractedOffsets, itdata/qutil.h:93-242).  Task t contracts the A
+// block at offs[3t] with the B block at offs[3t+1] into the C block at
+// offs[3t+2]; cfirst[t] marks the first task of its C block.
+struct QNTaskList {
+  IndexSet N;
+  std::vector<int32_t> la, lb, lc;
+  BlockOffsets coffs;
+  int64_t csize = 0;
+  int nt = 0;
+  std::vector<int64_t> offs, bda, bdb;
+  std::vector<int> cfirst;
+};
+QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet const &Bis,
+                      BlockOffsets const &Boffs);
+
+// A contraction A*B planned on the structure of its operands only (index
+// sets, storage kind, block lists).  The result structure and the device
+// plan are built once; exec() runs on any operands of the same structures
+// and writes every element of the result (first-touch).  With perBlock,
+// a QN contraction runs as one launch per block pair with the betas of
+// doTask(Contract,QDense,QDense) (itdata/qdense.cc:700-731) -- the
+// reference's per-block loop -- instead of one grouped launch.
+class ContractPlan {
+ public:
+  ContractPlan(ITensor const &A, ITensor const &B, int prec, bool perBlock = false);
+  ~ContractPlan();
+  ContractPlan(ContractPlan const &) = delete;
+  ContractPlan &operator=(ContractPlan const &) = delete;
+  IndexSet const &inds() const { return N_; }
+  bool isQN() const { return qn_; }
+  BlockOffsets const &offsets() const { return coffs_; }
+  size_t size() const { return size_; }
+  double flops() const { return flops_; }
+  itx_plan *plan() const { return plan_; }  // null in per-block mode
+  bool perBlock() const { return !blockPlans_.empty() || (perBlock_ && !plan_); }
+  std::shared_ptr<Storage> newOutput() const;
+  void exec(double const *A, double const *B, double *C) const;
+  // structure test of an operand against the planned one
+  static bool sameStructure(ITensor const &a, IndexSet const &is, bool qn, BlockOffsets const &offs);
+
+ private:
+  IndexSet N_;
+  bool qn_ = false, perBlock_ = false;
+  BlockOffsets coffs_;
+  size_t size_ = 0;
+  double flops_ = 0;
+  itx_plan *plan_ = nullptr;
+  std::vector<itx_plan *> blockPlans_;
+  std::vector<double> blockBeta_;
+};
 
 // ------------------------------------------------------------- decomps --
 struct Spectrum {

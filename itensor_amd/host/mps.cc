@@ -107,6 +107,7 @@ bool sameOrder(IndexSet const &a, IndexSet const &b) {
 }  // namespace
 
 bool LocalOp::setFused(ITensor const *W12, IndexSet const &phiInds) {
+  progs_.clear();
   W12_ = nullptr;
   if (!W12 || LIsNull() || RIsNull()) return false;
   IndexSet t1 = productInds(phiInds, L_->inds());
@@ -115,6 +116,115 @@ bool LocalOp::setFused(ITensor const *W12, IndexSet const &phiInds) {
   if (!sameOrder(ref, fus)) return false;
   W12_ = W12;
   return true;
+}
+
+// ------------------------------------------------------ planned products --
+// One product chain planned for one phi structure: the stage plans, the
+// intermediate buffers (reused by every product) and, in Graph mode, the
+// chain replayed as a HIP graph.
+struct LocalOp::Program {
+  IndexSet phiInds;
+  bool qn = false;
+  BlockOffsets phiOffs;
+  std::vector<ITensor const *> ops;
+  std::vector<std::unique_ptr<ContractPlan>> stages;
+  std::vector<DevPtr> tmp;  // outputs of every stage but the last
+  IndexSet outInds;
+  itx_chain *chain = nullptr;
+  double flops = 0;
+  ~Program() {
+    if (chain) itx_chain_destroy(ctx(), chain);
+  }
+};
+
+std::vector<ITensor const *> LocalOp::chainOps() const {
+  std::vector<ITensor const *> ops;
+  if (W12_ && isBulk()) return {L_, W12_, R_};
+  if (LIsNull()) {
+    if (!RIsNull()) ops.push_back(R_);
+    ops.push_back(Op2_);
+    ops.push_back(Op1_);
+  } else {
+    ops = {L_, Op1_, Op2_};
+    if (!RIsNull()) ops.push_back(R_);
+  }
+  return ops;
+}
+
+LocalOp::Program *LocalOp::program(ITensor const &phi) const {
+  if (mode_ == HpsiMode::Eager || !phi || phi.isCombiner() || phi.order() == 0) return nullptr;
+  for (auto it = progs_.rbegin(); it != progs_.rend(); ++it)
+    if (ContractPlan::sameStructure(phi, (*it)->phiInds, (*it)->qn, (*it)->phiOffs)) return it->get();
+  auto ops = chainOps();
+  for (auto *o : ops)
+    if (!o || !*o || o->isCombiner() || o->order() == 0 || o->isQN() != phi.isQN()) return nullptr;
+  auto prg = std::make_shared<Program>();
+  prg->phiInds = phi.inds();
+  prg->qn = phi.isQN();
+  if (prg->qn) prg->phiOffs = phi.store().offsets;
+  prg->ops = ops;
+  // structure-only stand-in for each stage's left operand
+  ITensor cur = phi;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    prg->stages.push_back(std::make_unique<ContractPlan>(cur, *ops[i], prec_, mode_ == HpsiMode::PerBlock));
+    ContractPlan const &cp = *prg->stages.back();
+    prg->flops += cp.flops();
+    auto st = std::make_shared<Storage>();
+    st->type = cp.isQN() ? SType::QDense : SType::Dense;
+    st->offsets = cp.offsets();
+    st->size = cp.size();
+    if (i + 1 < ops.size()) {
+      st->data = devAlloc(cp.size());
+      prg->tmp.push_back(st->data);
+    }
+    cur = ITensor(cp.inds(), st);
+  }
+  cur.noPrime();
+  prg->outInds = cur.inds();
+  if (mode_ == HpsiMode::Graph) {
+    std::vector<itx_plan *> plans;
+    for (auto &cp : prg->stages)
+      if (cp->plan()) plans.push_back(cp->plan());
+    if (plans.size() == prg->stages.size())
+      check(itx_chain_create(ctx(), int(plans.size()), plans.data(), &prg->chain), "itx_chain_create");
+  }
+  if (progs_.size() >= 4) progs_.erase(progs_.begin());
+  progs_.push_back(prg);
+  return prg.get();
+}
+
+void LocalOp::product(ITensor const &phi, ITensor &phip) const {
+  PrecisionScope ps(prec_);
+  if (Program *prg = program(phi)) {
+    const size_t n = prg->stages.size();
+    auto out = prg->stages.back()->newOutput();
+    std::vector<const double *> A(n), B(n);
+    std::vector<double *> C(n);
+    for (size_t i = 0; i < n; ++i) {
+      A[i] = i == 0 ? phi.store().data->p : prg->tmp[i - 1]->p;
+      B[i] = prg->ops[i]->store().data->p;
+      C[i] = i + 1 == n ? out->data->p : prg->tmp[i]->p;
+    }
+    if (prg->chain) {
+      check(itx_chain_launch(ctx(), prg->chain, A.data(), B.data(), C.data(), nullptr, nullptr),
+            "itx_chain_launch");
+    } else {
+      for (size_t i = 0; i < n; ++i) prg->stages[i]->exec(A[i], B[i], C[i]);
+    }
+    phip = ITensor(prg->outInds, out);
+    return;
+  }
+  auto ops = chainOps();
+  phip = phi;
+  for (auto *o : ops) phip *= *o;
+  phip.noPrime();
+}
+
+double LocalOp::flops(ITensor const &phi) const {
+  PrecisionScope ps(prec_);
+  Program *prg = program(phi);
+  if (!prg) throw ITError("LocalOp::flops: chain cannot be planned for this phi (Eager mode or mixed storage)");
+  return prg->flops;
 }
 
 void LocalMPO::fuseOps(IndexSet const &phiInds) {
@@ -188,6 +298,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   LocalMPO PH(H);
   PH.setPrecision(int(args.getInt("HpsiPrecision", ITX_PREC_F64)));
   PH.setFusion(args.getBool("FuseMPO", true));
+  PH.setMode(HpsiMode(args.getInt("HpsiMode", long(HpsiMode::Graph))));
   Real energy = NAN;
   for (size_t sw = 0; sw < sweeps.size(); ++sw) {
     auto const &S = sweeps[sw];

@@ -36,6 +36,18 @@ class MPS {
 
 using MPO = std::vector<ITensor>;  // W_1..W_N (0-based storage)
 
+// How LocalOp::product runs its contraction chain.
+//   Eager    : ITensor operator* per stage (planned on every call, plans
+//              cached by structure inside libitx)
+//   Planned  : the chain is planned once per phi structure (result
+//              structures, device task lists, intermediate buffers) and
+//              its stage plans are executed directly
+//   Graph    : as Planned, replayed as one HIP graph (itx_chain)
+//   PerBlock : as Planned, but each QN block pair is its own launch (the
+//              per-block loop of doTask(Contract,QDense,QDense),
+//              itdata/qdense.cc:700-731) -- for comparison
+enum class HpsiMode { Eager = 0, Planned = 1, Graph = 2, PerBlock = 3 };
+
 // The two-site projected Hamiltonian acting on phi (mps/localop.h).
 class LocalOp {
  public:
@@ -46,46 +58,41 @@ class LocalOp {
     L_ = L;
     R_ = R;
     W12_ = nullptr;
+    progs_.clear();
   }
   bool LIsNull() const { return L_ == nullptr || !(*L_); }
   bool isBulk() const { return !LIsNull() && !RIsNull(); }
   bool RIsNull() const { return R_ == nullptr || !(*R_); }
-  // product (localop.h:324-365), two centre sites; contractions run in the
-  // precision form `prec` (ITX_PREC_*), everything else stays FP64
-  void product(ITensor const &phi, ITensor &phip) const {
-    PrecisionScope ps(prec_);
-    if (W12_ && !LIsNull() && !RIsNull()) {
-      // bulk case with the two MPO tensors pre-contracted (same result
-      // index order as the four-stage chain, checked in setFused)
-      phip = phi * (*L_);
-      phip *= *W12_;
-      phip *= *R_;
-      phip.noPrime();
-      return;
-    }
-    if (LIsNull()) {
-      phip = phi;
-      if (!RIsNull()) phip *= *R_;
-      phip *= *Op2_;
-      phip *= *Op1_;
-    } else {
-      phip = phi * (*L_);
-      phip *= *Op1_;
-      phip *= *Op2_;
-      if (!RIsNull()) phip *= *R_;
-    }
-    phip.noPrime();
-  }
+  // product (localop.h:324-365): phip = noPrime(phi*L*Op1*Op2*R) for the
+  // two centre sites (the L-null and R-null ends drop that factor); the
+  // contractions run in the precision form `prec` (ITX_PREC_*)
+  void product(ITensor const &phi, ITensor &phip) const;
   long size() const;  // localop.h:488-530
-  void setPrecision(int p) { prec_ = p; }
+  void setPrecision(int p) {
+    if (p != prec_) progs_.clear();
+    prec_ = p;
+  }
+  void setMode(HpsiMode m) {
+    if (m != mode_) progs_.clear();
+    mode_ = m;
+  }
+  HpsiMode mode() const { return mode_; }
   // Use W12 = Op1*Op2 in the bulk case when ((phi*L)*W12)*R yields exactly
   // the index order of ((phi*L)*Op1)*Op2*R; returns whether it does.
   bool setFused(ITensor const *W12, IndexSet const &phiInds);
+  // algorithmic flops (sum of 2mnk over every block pair) of one product
+  // on a tensor with phi's structure (plans the chain if needed)
+  double flops(ITensor const &phi) const;
 
  private:
+  struct Program;
+  std::vector<ITensor const *> chainOps() const;
+  Program *program(ITensor const &phi) const;
   ITensor const *Op1_ = nullptr, *Op2_ = nullptr, *L_ = nullptr, *R_ = nullptr;
   ITensor const *W12_ = nullptr;
   int prec_ = ITX_PREC_F64;
+  HpsiMode mode_ = HpsiMode::Graph;
+  mutable std::vector<std::shared_ptr<Program>> progs_;  // most recent last
 };
 
 class LocalMPO {
@@ -98,6 +105,8 @@ class LocalMPO {
   ITensor const &L() const { return PH_.at(size_t(LHlim_)); }
   ITensor const &R() const { return PH_.at(size_t(RHlim_)); }
   void setPrecision(int p) { lop_.setPrecision(p); }
+  void setMode(HpsiMode m) { lop_.setMode(m); }
+  double flops(ITensor const &phi) const { return lop_.flops(phi); }
   double envSeconds() const { return env_seconds_; }
   // pre-contract the two MPO tensors of the current bond for phi's index set
   void fuseOps(IndexSet const &phiInds);

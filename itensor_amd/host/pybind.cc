@@ -75,6 +75,24 @@ py::dict recordDict(BondRecord const &r) {
   return d;
 }
 
+// LocalOp bound to tensors it co-owns (the reference's LocalOp holds
+// non-owning pointers into LocalMPO, localop.h:46-49)
+struct PyLocalOp {
+  ITensor W1, W2, L, R, W12;
+  bool fuse = true, fuseChecked = false;
+  LocalOp op;
+  PyLocalOp(ITensor w1, ITensor w2, ITensor l, ITensor r, bool f)
+      : W1(std::move(w1)), W2(std::move(w2)), L(std::move(l)), R(std::move(r)), fuse(f) {
+    op.update(&W1, &W2, &L, &R);
+    if (fuse && op.isBulk()) W12 = W1 * W2;
+  }
+  void prepare(ITensor const &phi) {
+    if (fuseChecked) return;
+    if (fuse && op.isBulk()) op.setFused(&W12, phi.inds());
+    fuseChecked = true;
+  }
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_itensor_host, m) {
@@ -180,18 +198,20 @@ PYBIND11_MODULE(_itensor_host, m) {
     return py::make_tuple(e, phi, st.products);
   }, "davidson on the two-site problem of bond b (iterativesolvers_test.cc:43-65 pattern)");
   m.def("local_product", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, int b, ITensor const &phi,
-                            int prec, bool fuse) {
+                            int prec, bool fuse, int mode) {
     MPO mpo(H);
     MPS psi(A);
     LocalMPO PH(mpo);
     PH.position(b, psi);
     PH.setPrecision(prec);
     PH.setFusion(fuse);
+    PH.setMode(HpsiMode(mode));
     PH.fuseOps(phi.inds());
     ITensor out;
     PH.product(phi, out);
     return out;
-  }, py::arg("H"), py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("prec") = 0, py::arg("fuse") = true);
+  }, py::arg("H"), py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("prec") = 0, py::arg("fuse") = true,
+     py::arg("mode") = int(HpsiMode::Graph));
   m.def("dmrg", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, py::list sweeps, py::dict args) {
     MPS psi(A);
     std::vector<py::dict> recs;
@@ -200,6 +220,41 @@ PYBIND11_MODULE(_itensor_host, m) {
                     [&](BondRecord const &r) { recs.push_back(recordDict(r)); }, &secs);
     return py::make_tuple(e, recs, psi.tensors(), secs);
   });
+  py::class_<PyLocalOp>(m, "LocalOp")
+      .def(py::init<ITensor, ITensor, ITensor, ITensor, bool>(), py::arg("W1"), py::arg("W2"), py::arg("L"),
+           py::arg("R"), py::arg("fuse") = true,
+           "LocalOp(Op1, Op2, L, R) (localop.h:114-127); an empty ITensor() for L or R marks that end")
+      .def("product", [](PyLocalOp &o, ITensor const &phi) {
+        o.prepare(phi);
+        ITensor out;
+        o.op.product(phi, out);
+        return out;
+      })
+      .def("flops", [](PyLocalOp &o, ITensor const &phi) {
+        o.prepare(phi);
+        return o.op.flops(phi);
+      })
+      .def("size", [](PyLocalOp &o) { return o.op.size(); })
+      .def("set_precision", [](PyLocalOp &o, int p) { o.op.setPrecision(p); })
+      .def("set_mode", [](PyLocalOp &o, int m) {
+        if (m < 0 || m > 3) throw ITError("LocalOp.set_mode: mode must be 0..3");
+        o.op.setMode(HpsiMode(m));
+      })
+      .def_property_readonly("fused", [](PyLocalOp const &o) { return bool(o.W12); });
+  m.attr("HPSI_EAGER") = int(HpsiMode::Eager);
+  m.attr("HPSI_PLANNED") = int(HpsiMode::Planned);
+  m.attr("HPSI_GRAPH") = int(HpsiMode::Graph);
+  m.attr("HPSI_PER_BLOCK") = int(HpsiMode::PerBlock);
+  m.def("random_itensor", [](std::vector<Index> const &is, QNList const &div, uint64_t seed) {
+    return randomITensor(qnFrom(div), is, seed);
+  }, py::arg("inds"), py::arg("div") = QNList(), py::arg("seed") = 1);
+  m.def("from_dense", [](std::vector<Index> const &is, py::array_t<double, py::array::forcecast | py::array::f_style> a,
+                         QNList const &div) {
+    std::vector<double> v(a.data(), a.data() + a.size());
+    if (!hasQNs(is)) return denseFromHost(is, v);
+    return qdenseFromDense(is, v, qnFrom(div));
+  }, py::arg("inds"), py::arg("array"), py::arg("div") = QNList(),
+     "ITensor from a dense array (column-major); QDense of flux div when the indices carry QNs");
   m.def("set_stream", [](uintptr_t s) { setStream(reinterpret_cast<void *>(s)); });
   m.def("synchronize", []() { check(itx_synchronize(ctx()), "itx_synchronize"); });
   m.def("set_contract_precision", &setContractPrecision);

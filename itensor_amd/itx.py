@@ -71,6 +71,9 @@ _PROTOS = {
     "itx_tune_save": (C.c_int, [_vp, C.c_char_p]),
     "itx_tune_load": (C.c_int, [_vp, C.c_char_p]),
     "itx_tune_size": (C.c_int, [_vp]),
+    "itx_chain_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
+    "itx_chain_launch": (C.c_int, [_vp, _vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]),
+    "itx_chain_destroy": (C.c_int, [_vp, _vp]),
 }
 
 
@@ -187,6 +190,12 @@ class Context:
         self.check(rc, "itx_plan_qn")
         return Plan(self, h)
 
+    def chain(self, plans: list["Plan"]) -> "Chain":
+        arr = (_vp * len(plans))(*[p.h for p in plans])
+        h = _vp()
+        self.check(self.L.itx_chain_create(self.h, len(plans), arr, C.byref(h)), "itx_chain_create")
+        return Chain(self, h, plans)
+
     # ---- tile autotuner ----
     def set_autotune(self, on: bool = True):
         self.check(self.L.itx_set_autotune(self.h, 1 if on else 0), "itx_set_autotune")
@@ -265,6 +274,36 @@ class Plan:
     def close(self):
         if self.h:
             self.ctx.L.itx_plan_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Chain:
+    """Plan executions replayed as one HIP graph (itx_chain_*)."""
+
+    def __init__(self, ctx: Context, h, plans):
+        self.ctx, self.h, self.plans = ctx, h, list(plans)
+
+    def launch(self, A: list[int], B: list[int], Cp: list[int], alpha=None, beta=None):
+        n = len(self.plans)
+        pa = (_vp * n)(*[_vp(x) for x in A])
+        pb = (_vp * n)(*[_vp(x) for x in B])
+        pc = (_vp * n)(*[_vp(x) for x in Cp])
+        al = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        be = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64)
+        self.ctx.check(self.ctx.L.itx_chain_launch(
+            self.ctx.h, self.h, pa, pb, pc,
+            None if al is None else al.ctypes.data_as(_dp),
+            None if be is None else be.ctypes.data_as(_dp)), "itx_chain_launch")
+
+    def close(self):
+        if self.h:
+            self.ctx.L.itx_chain_destroy(self.ctx.h, self.h)
             self.h = None
 
     def __del__(self):

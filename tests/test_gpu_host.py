@@ -171,9 +171,11 @@ def test_davidson_closed_form_gpu(H, qn):
 
 
 @pytest.mark.parametrize("fuse", [False, True])
-def test_local_product_matches_oracle_dense(H, fuse):
-    """LocalOp::product (four-stage chain, or with the MPO pair fused) vs the
-    oracle's four-stage chain: same result index order, values 1e-12."""
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_local_product_matches_oracle_dense(H, fuse, mode):
+    """LocalOp::product (four-stage chain, or with the MPO pair fused; every
+    execution mode) vs the oracle's four-stage chain, at both chain ends and
+    in the bulk: same result index order, values 1e-12."""
     sites = OD.spin_half_sites(8, False)
     Hm = OD.heisenberg_mpo(sites)
     psi = OD.random_mps(sites, 16, seed=11)
@@ -182,13 +184,15 @@ def test_local_product_matches_oracle_dense(H, fuse):
         PH.position(b, psi)
         phi = psi(b) * psi(b + 1)
         ref = PH.product(phi)
-        got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi), 0, fuse)
+        got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi), 0, fuse, mode)
         assert same_inds(got, cten(ref))
         assert _rel(got.store(), ref.d.ravel(order="F")) < 1e-12
 
 
-def test_local_product_qn_matches_oracle(H):
-    """QN LocalOp::product: block list bit-exact, values 1e-12."""
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_local_product_qn_matches_oracle(H, mode):
+    """QN LocalOp::product at both chain ends and in the bulk (eager, graph
+    and per-block modes): block list bit-exact, values 1e-12."""
     N = 10
     sites = OD.spin_half_sites(N, True)
     Hm = OD.heisenberg_mpo(sites)
@@ -200,7 +204,7 @@ def test_local_product_qn_matches_oracle(H):
         phi = psi(b) * psi(b + 1)
         ref = PH.product(phi)
         for fuse in (False, True):
-            got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi), 0, fuse)
+            got = H.local_product([cten(w) for w in Hm], [cten(a) for a in psi.A], b, cten(phi), 0, fuse, mode)
             assert same_inds(got, cten(ref))
             assert [(tuple(x), o) for x, o in got.offsets] == [(x, o) for x, o in ref.offsets]
             assert _rel(got.store(), ref.store) < 1e-12
@@ -227,7 +231,10 @@ def test_dmrg_config1_trajectory_matches_oracle(H):
     print(f"config-1 sec/sweep (GPU FP64): {secs}")
 
 
-def test_qn_dmrg_matches_oracle_and_ed(H):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_qn_dmrg_matches_oracle_and_ed(H, mode):
+    """QN DMRG N=12 (eager products, and the planned chain replayed as a HIP
+    graph): per-bond energies and kept dims as the oracle, energy vs ED."""
     N = 12
     sites = OD.spin_half_sites(N, True)
     Hm = OD.heisenberg_mpo(sites)
@@ -235,7 +242,7 @@ def test_qn_dmrg_matches_oracle_and_ed(H):
     sw = [{"MaxDim": m, "Cutoff": 1e-12, "Niter": 2} for m in (10, 20, 40, 60, 60)]
     trace = []
     Eo, _ = OD.dmrg(Hm, psi0, sw, record=trace.append)
-    Eg, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {})
+    Eg, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {"HpsiMode": mode})
     assert [r["dim"] for r in recs] == [t["dim"] for t in trace]
     assert max(abs(r["energy"] - t["energy"]) for r, t in zip(recs, trace)) < 1e-10
     assert abs(Eg - GOLD["energies"]["12"]) < 1e-9
