@@ -190,6 +190,81 @@ def qn_hpsi(steps: int, warmup: int):
     return res
 
 
+def heisenberg_qn_chain(H, N: int):
+    """Sz-conserving spin-1/2 Heisenberg chain (J = Jz = 1, open ends):
+    site indices, the MPO (bulk W(dag(s), s', dag(h_in), h_out), first and
+    last site contracted with the start / end states of the MPO link,
+    sample/Heisenberg.h:76-116) and the Neel product state with link QNs
+    that give every site tensor flux 0."""
+    def sz(v):
+        return [("Sz", int(v))]
+
+    ids = iter(range(100001, 100001 + 4 * N + 8))
+
+    def idx(qns, tags):
+        return H.Index(next(ids), sum(d for _, d in qns), 1, 0, tags, qns)
+
+    sites = [idx([(sz(1), 1), (sz(-1), 1)], f"Site,S=1/2,n={n}") for n in range(1, N + 1)]
+    hq = [(sz(0), 3), (sz(-2), 1), (sz(2), 1)]
+    hl = [idx(hq, f"Link,H,l={n}") for n in range(N + 1)]
+    Wb = heisenberg_w().transpose(1, 2, 0, 3)  # (s, s', h_in, h_out)
+    mpo = []
+    for n in range(1, N + 1):
+        s, row, col = sites[n - 1], hl[n - 1].dag(), hl[n]
+        if n == 1:
+            W, inds = Wb[:, :, 1, :], [s.dag(), s.prime(), col]
+        elif n == N:
+            W, inds = Wb[:, :, :, 0], [s.dag(), s.prime(), row]
+        else:
+            W, inds = Wb, [s.dag(), s.prime(), row, col]
+        mpo.append(H.from_dense(inds, np.ascontiguousarray(W)))
+    # Neel state: link l_n carries -(sum of the site charges 1..n)
+    A, q, prev = [], 0, None
+    for n in range(1, N + 1):
+        st = (n - 1) % 2  # 0 = Up (Sz +1), 1 = Dn
+        q -= 1 if st == 0 else -1
+        s = sites[n - 1]
+        v = np.zeros(2)
+        v[st] = 1.0
+        if n < N:
+            link = idx([(sz(q), 1)], f"Link,l={n}")
+        if n == 1:
+            A.append(H.from_dense([s, link], v.reshape(2, 1)))
+        elif n == N:
+            A.append(H.from_dense([prev.dag(), s], v.reshape(1, 2)))
+        else:
+            A.append(H.from_dense([prev.dag(), s, link], v.reshape(1, 2, 1)))
+        prev = link if n < N else None
+    return mpo, A
+
+
+DMRG_MAXDIM = [10, 20, 50, 100, 200, 400, 800, 1200, 1600, 2048]
+
+
+def dmrg_leg(N: int, maxdims, prec: int, mode: int):
+    """QN Heisenberg DMRG (N sites, Neel start, one sweep per maxdim entry,
+    cutoff 1e-10, Davidson niter 2) in one H*psi form; wall time per sweep
+    (everything included) and the phase split of each sweep."""
+    import torch
+
+    from itensor_amd import host as H
+
+    mpo, A = heisenberg_qn_chain(H, N)
+    sweeps = [{"MaxDim": int(m), "Cutoff": 1e-10, "Niter": 2} for m in maxdims]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "HpsiMode": mode, "SyncTiming": True})
+    wall = time.perf_counter() - t0
+    per = []
+    for k in range(len(maxdims)):
+        rs = [r for r in recs if r["sweep"] == k + 1]
+        per.append({"maxdim": int(maxdims[k]), "sec": round(secs[k], 4), "dim": max(r["dim"] for r in rs),
+                    "env": round(sum(r["t_env"] for r in rs), 4), "davidson": round(sum(r["t_davidson"] for r in rs), 4),
+                    "svd": round(sum(r["t_svd"] for r in rs), 4), "products": sum(r["nprod"] for r in rs),
+                    "truncerr_max": max(r["truncerr"] for r in rs), "energy": rs[-1]["energy"]})
+    return {"energy": E, "wall_s": round(wall, 3), "sweeps": per}
+
+
 def copy_calibration(gib: float):
     """A plain device-to-device copy of known size (reads and writes
     `gib` GiB per launch), the reference point for HBM byte counters."""
@@ -217,6 +292,8 @@ def main():
     ap.add_argument("--chain", default="fused", choices=["fused", "4stage"],
                     help="LocalOp::product as run by the host layer (MPO pair fused) or the 4-stage chain")
     ap.add_argument("--no-qn", action="store_true", help="skip the block-sparse (31 Sz sectors) H*psi leg")
+    ap.add_argument("--dmrg", default="", help="run the QN DMRG leg: N:maxdim1,maxdim2,... (e.g. 100:full)")
+    ap.add_argument("--dmrg-forms", default="f64,f32", help="H*psi forms of the DMRG leg")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
@@ -340,6 +417,19 @@ def main():
         q = qn_hpsi(max(5, args.steps), args.warmup)
         line["qn_hpsi"] = {"workload": "block-sparse two-site Hpsi, chi=1024 over 31 Sz sectors "
                                        "(max 120, min 1), site {+1,-1}, MPO link {0:3,-2:1,+2:1}", **q}
+
+    if args.dmrg:
+        n_s, md_s = args.dmrg.split(":")
+        mds = DMRG_MAXDIM if md_s == "full" else [int(x) for x in md_s.split(",")]
+        line["dmrg"] = {"workload": "QN Heisenberg N=%s open chain, Neel start, maxdim %s, cutoff 1e-10, niter 2"
+                                    % (n_s, mds)}
+        for form in args.dmrg_forms.split(","):
+            line["dmrg"][form] = dmrg_leg(int(n_s), mds, {"f64": 0, "f32": 1}[form], 2)
+            print("dmrg leg %s done in %.1f s" % (form, line["dmrg"][form]["wall_s"]), file=sys.stderr, flush=True)
+        if "f64" in line["dmrg"] and "f32" in line["dmrg"]:
+            a, b = line["dmrg"]["f64"], line["dmrg"]["f32"]
+            line["dmrg"]["dE_f32_vs_f64"] = b["energy"] - a["energy"]
+            line["dmrg"]["speedup_f32_vs_f64"] = round(a["wall_s"] / b["wall_s"], 3)
 
     if not args.no_cpu:
         ref, t_cpu, threads, reps = cpu_baseline(psi, L, W, R)

@@ -331,6 +331,121 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny(const double *__restrict__ A
   }
 }
 
+// Small-block contractions (C blocks of a few thousand elements: the QN
+// blocks of DMRG at moderate bond dimension): latency-bound, so FP64 VALU
+// arithmetic in every form.  A unit is a 64 x 32 tile of one C block; for
+// each task and 64-wide k chunk the A tile and the B tile are staged
+// cooperatively in LDS -- one round of independent loads, lanes packed
+// along the operand's unit stride -- then each thread accumulates 8
+// outputs of one row.  Same task/unit/table format as k_gemm.
+constexpr int SM_M = 64, SM_N = 32, SM_K = 64, SM_T = 256, SM_NPT = 8;
+constexpr int SM_LDA = SM_M + 1;  // pad: the k-contiguous staging stores hit distinct banks
+
+template <bool AKC, bool BKC>
+__global__ __launch_bounds__(SM_T) void k_small(const double *__restrict__ A, const double *__restrict__ B,
+                                                double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                const WorkUnit *__restrict__ units, int nunits,
+                                                const int64_t *__restrict__ tab, double alpha, double beta) {
+  __shared__ double As[SM_K * SM_LDA];
+  __shared__ double Bs[SM_K * SM_N];
+  const int tid = threadIdx.x;
+  const int mi = tid & (SM_M - 1), ng = tid >> 6;  // output row, group of 8 columns
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const WorkUnit wu = units[u];
+    double acc[SM_NPT];
+#pragma unroll
+    for (int j = 0; j < SM_NPT; ++j) acc[j] = 0.0;
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      const int rows = min(SM_M, tk.M - wu.m0), cols = min(SM_N, tk.N - wu.n0);
+      for (int k0 = 0; k0 < tk.K; k0 += SM_K) {
+        const int kc = min(SM_K, tk.K - k0);
+        __syncthreads();  // the previous chunk's tiles are no longer read
+        // A tile (rows x kc) -> As[k][m]; consecutive lanes follow A's
+        // unit stride (k when A is k-contiguous, else m), packed densely
+        {
+          const int na = rows * kc, lead = AKC ? kc : rows;
+          for (int e = tid; e < na; e += SM_T) {
+            const int q = e / lead, r = e - q * lead;
+            const int m = AKC ? q : r, k = AKC ? r : q;
+            As[k * SM_LDA + m] = A[tk.a_off + tab[tk.tAM + wu.m0 + m] + tab[tk.tAK + k0 + k]];
+          }
+        }
+        // B tile (kc x cols) -> Bs[k][n], likewise
+        {
+          const int nb = kc * cols, lead = BKC ? kc : cols;
+          for (int e = tid; e < nb; e += SM_T) {
+            const int q = e / lead, r = e - q * lead;
+            const int n = BKC ? q : r, k = BKC ? r : q;
+            Bs[k * SM_N + n] = B[tk.b_off + tab[tk.tBK + k0 + k] + tab[tk.tBN + wu.n0 + n]];
+          }
+          // columns beyond the block are read by the last column group
+          for (int e = tid; e < kc * (SM_N - cols); e += SM_T) {
+            const int k = e / (SM_N - cols), n = cols + (e - k * (SM_N - cols));
+            Bs[k * SM_N + n] = 0.0;
+          }
+        }
+        __syncthreads();
+        const double *bp = Bs + ng * SM_NPT;
+        for (int k = 0; k < kc; ++k) {
+          const double a = As[k * SM_LDA + mi];
+#pragma unroll
+          for (int j = 0; j < SM_NPT; ++j) acc[j] = fma(a, bp[k * SM_N + j], acc[j]);
+        }
+      }
+    }
+    if (wu.slot >= 0) {  // partial tile of a split unit (C is the workspace)
+      double *P = C + size_t(wu.slot) * (SM_M * SM_N);
+#pragma unroll
+      for (int j = 0; j < SM_NPT; ++j) P[(ng * SM_NPT + j) * SM_M + mi] = acc[j];
+      continue;
+    }
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int m = wu.m0 + mi;
+    if (m < t0.M) {
+      double *Cm = C + t0.c_off + tab[t0.tCM + m];
+#pragma unroll
+      for (int j = 0; j < SM_NPT; ++j) {
+        const int n = wu.n0 + ng * SM_NPT + j;
+        if (n < t0.N && ng * SM_NPT + j < SM_N) {
+          double *cp = Cm + tab[t0.tCN + n];
+          double v = alpha * acc[j];
+          if (beta != 0.0) v += beta * (*cp);
+          *cp = v;
+        }
+      }
+    }
+  }
+}
+
+static_assert(SM_T_RED == SM_T, "reduction pass block size");
+
+// Reduction pass of split plans: unit u = {task (for C's tables), first
+// slot, m0, n0, end slot}; C tile = alpha * (sum of the slots in order)
+// + beta * C.  A is the workspace; deterministic (fixed summation order).
+__global__ __launch_bounds__(SM_T) void k_reduce_tiles(const double *__restrict__ A,
+                                                       const double *__restrict__ B, double *__restrict__ C,
+                                                       const GemmTask *__restrict__ tasks,
+                                                       const WorkUnit *__restrict__ units, int nunits,
+                                                       const int64_t *__restrict__ tab, double alpha,
+                                                       double beta) {
+  (void)B;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const WorkUnit ru = units[u];
+    const GemmTask t0 = tasks[ru.tbeg];
+    for (int e = threadIdx.x; e < SM_M * SM_N; e += SM_T) {
+      const int n = e / SM_M, m = e - n * SM_M;
+      if (ru.m0 + m >= t0.M || ru.n0 + n >= t0.N) continue;
+      double s = 0;
+      for (int q = ru.tend; q < ru.slot; ++q) s += A[size_t(q) * (SM_M * SM_N) + e];
+      double *cp = C + t0.c_off + tab[t0.tCM + ru.m0 + m] + tab[t0.tCN + ru.n0 + n];
+      double v = alpha * s;
+      if (beta != 0.0) v += beta * (*cp);
+      *cp = v;
+    }
+  }
+}
+
 // ---- host side --------------------------------------------------------
 
 struct Digit {
@@ -443,7 +558,7 @@ void colmajor_strides(int r, const int64_t *d, int64_t *s) {
   }
 }
 
-// MFMA tiles (BM x BN with GM x GN waves) and the skinny VALU path.
+// MFMA tiles (BM x BN with GM x GN waves) and the small-block VALU path.
 struct TileCfg {
   int bm, bn, gm, gn;
   double eff64, eff32;  // relative per-CU efficiency (MI355X measurements)
@@ -454,14 +569,16 @@ constexpr TileCfg kTiles[] = {
     {64, 128, 2, 2, 0.92, 0.95},   // 2
     {64, 64, 2, 2, 1.00, 0.95},    // 3
     {32, 32, 2, 2, 0.50, 0.45},    // 4
-    {SK_ROWS, SK_N, 0, 0, 1.0, 1.0},  // 5: skinny
+    {SK_ROWS, SK_N, 0, 0, 1.0, 1.0},  // 5: skinny (FP64 VALU, large M)
     {128, 128, 2, 4, 0.80, 0.85},  // 6: 8 waves
     {256, 64, 4, 2, 0.70, 0.55},   // 7: 8 waves
     {64, 256, 2, 4, 0.95, 1.00},   // 8: 8 waves
     {256, 128, 4, 2, 0.60, 0.50},  // 9: 8 waves, wave tile 64x64
+    {SM_M, SM_N, 0, 0, 1.0, 1.0},  // 10: small blocks (FP64 VALU)
 };
-constexpr int kNumTiles = 10;
+constexpr int kNumTiles = 11;
 constexpr int kSkinny = 5;
+constexpr int kSmall = 10;
 constexpr int kSkinnyOcc = 4;
 
 // Kernel instantiation of a tile shape, operand layout and form.  All
@@ -495,6 +612,13 @@ const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block) {
   if (tile == kSkinny) {  // HBM-bound: FP64 arithmetic in every form
     *block = SK_ROWS;
     return reinterpret_cast<const void *>(&k_skinny);
+  }
+  if (tile == kSmall) {  // latency-bound: FP64 arithmetic in every form
+    *block = SM_T;
+    if (akc) return bkc ? reinterpret_cast<const void *>(&k_small<true, true>)
+                        : reinterpret_cast<const void *>(&k_small<true, false>);
+    return bkc ? reinterpret_cast<const void *>(&k_small<false, true>)
+               : reinterpret_cast<const void *>(&k_small<false, false>);
   }
   *block = 64 * kTiles[tile].gm * kTiles[tile].gn;
   return prec == ITX_PREC_F64 ? gemm_fn_tile<double>(tile, akc, bkc) : gemm_fn_tile<float>(tile, akc, bkc);
@@ -687,14 +811,60 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   auto finalize = [&](int best, itx_plan **pout) -> int {
     *pout = nullptr;
     const int tbm = kTiles[best].bm, tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
-    std::vector<WorkUnit> units;
+    std::vector<WorkUnit> units, red;
     std::vector<double> ucost;
-    for (auto &g : groups) {
-      for (int m0 = 0; m0 < g.M; m0 += tbm)
-        for (int n0 = 0; n0 < g.N; n0 += tbn) {
-          units.push_back({g.tb, g.te, m0, n0});
-          ucost.push_back(g.ksum);
+    std::vector<GemmTask> vt;  // device task list (st, or its k-chunks when split)
+    int nslots = 0;
+    bool split = false;
+    if (best == kSmall) {
+      // Split the k-chunk sequence of heavy tiles over several units when
+      // the tiles alone cannot fill the device (few C blocks, long task
+      // lists): chunk budget c per unit from the total work.
+      double total = 0, maxw = 0;
+      for (auto &g : groups) {
+        double w = 0;
+        for (int t = g.tb; t < g.te; ++t) w += double((st[t].K + SM_K - 1) / SM_K);
+        const double tiles = double((g.M + tbm - 1) / tbm) * double((g.N + tbn - 1) / tbn);
+        total += w * tiles;
+        maxw = std::max(maxw, w);
+      }
+      const double target = 2.0 * cus * tile_occupancy(kSmall, prec, akc, bkc);
+      const double c = std::max(1.0, std::ceil(total / target));
+      split = maxw > 2.0 * c && total * (SM_M * SM_N) * 8.0 / c < double(size_t(1) << 30);
+      if (split) {
+        const int per = int(c);
+        for (auto &g : groups) {
+          const int vb = int(vt.size());
+          for (int t = g.tb; t < g.te; ++t)
+            for (int k0 = 0; k0 < st[t].K; k0 += SM_K) {
+              GemmTask x = st[t];
+              x.K = std::min(SM_K, st[t].K - k0);
+              x.tAK += k0;
+              x.tBK += k0;
+              vt.push_back(x);
+            }
+          const int ve = int(vt.size());
+          for (int m0 = 0; m0 < g.M; m0 += tbm)
+            for (int n0 = 0; n0 < g.N; n0 += tbn) {
+              const int s0 = nslots;
+              for (int a = vb; a < ve; a += per) {
+                units.push_back({a, std::min(ve, a + per), m0, n0, nslots++, 0});
+                ucost.push_back(double(std::min(ve, a + per) - a));
+              }
+              red.push_back({vb, s0, m0, n0, nslots, 0});
+            }
         }
+      }
+    }
+    if (!split) {
+      vt = st;
+      for (auto &g : groups) {
+        for (int m0 = 0; m0 < g.M; m0 += tbm)
+          for (int n0 = 0; n0 < g.N; n0 += tbn) {
+            units.push_back({g.tb, g.te, m0, n0, -1, 0});
+            ucost.push_back(g.ksum);
+          }
+      }
     }
     // heaviest units first (the persistent loop then balances like LPT);
     // stable so equal-cost tiles keep their m-major order
@@ -711,41 +881,54 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->tile = best;
     p->a_kcontig = akc;
     p->b_kcontig = bkc;
-    p->ntasks = int(st.size());
+    p->ntasks = int(vt.size());
     p->nunits = int(units.size());
     p->flops = flops;
     p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
     p->fn = kernel_of(best, prec, akc, bkc, &p->block);
+    p->nred = int(red.size());
+    if (p->nred) {
+      p->fn_red = reinterpret_cast<const void *>(&k_reduce_tiles);
+      p->grid_red = int(std::min<int64_t>(int64_t(red.size()), int64_t(cus) * 8));
+    }
     if (std::getenv("ITX_PLAN_LOG"))
-      std::fprintf(stderr, "[itx plan] tasks=%d units=%d tile=%dx%d%s grid=%d akc=%d bkc=%d flops=%.3e\n",
-                   p->ntasks, p->nunits, tbm, tbn, best == kSkinny ? "(skinny)" : "", p->grid, int(akc), int(bkc),
-                   flops);
+      std::fprintf(stderr, "[itx plan] tasks=%d units=%d tile=%dx%d%s grid=%d akc=%d bkc=%d flops=%.3e split=%d\n",
+                   p->ntasks, p->nunits, tbm, tbn, best == kSkinny ? "(skinny)" : best == kSmall ? "(small)" : "",
+                   p->grid, int(akc), int(bkc), flops, p->nred);
     if (p->nunits > 0) {
-      size_t bt_bytes = st.size() * sizeof(GemmTask);
+      size_t bt_bytes = vt.size() * sizeof(GemmTask);
       size_t bu_bytes = units.size() * sizeof(WorkUnit);
+      size_t br_bytes = red.size() * sizeof(WorkUnit);
       size_t o_units = (bt_bytes + 255) & ~size_t(255);
-      size_t o_tab = (o_units + bu_bytes + 255) & ~size_t(255);
-      size_t total = o_tab + tab.size() * sizeof(int64_t);
-      std::vector<char> host(total, 0);
-      std::memcpy(host.data(), st.data(), bt_bytes);
+      size_t o_red = (o_units + bu_bytes + 255) & ~size_t(255);
+      size_t o_tab = (o_red + br_bytes + 255) & ~size_t(255);
+      size_t o_ws = (o_tab + tab.size() * sizeof(int64_t) + 255) & ~size_t(255);
+      size_t total = o_ws + sizeof(double) * size_t(nslots) * (SM_M * SM_N);
+      std::vector<char> host(o_ws, 0);
+      std::memcpy(host.data(), vt.data(), bt_bytes);
       std::memcpy(host.data() + o_units, units.data(), bu_bytes);
+      if (br_bytes) std::memcpy(host.data() + o_red, red.data(), br_bytes);
       if (!tab.empty()) std::memcpy(host.data() + o_tab, tab.data(), tab.size() * sizeof(int64_t));
-      hipError_t e = hipMalloc(&p->dev, total);
-      if (e != hipSuccess) {
+      int rc3 = pool_alloc(ctx, total, &p->dev);
+      if (rc3) {
         delete p;
-        return hip_err(ctx, e, "hipMalloc(plan)");
+        return rc3;
       }
-      e = hipMemcpyAsync(p->dev, host.data(), total, hipMemcpyHostToDevice, ctx->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-      if (e != hipSuccess) {
-        (void)hipFree(p->dev);
+      p->ctx = ctx;
+      rc3 = stage_upload(ctx, p->dev, host.data(), o_ws);
+      if (rc3) {
+        pool_free(ctx, p->dev);
         delete p;
-        return hip_err(ctx, e, "upload(plan)");
+        return rc3;
       }
       p->dev_bytes = total;
       p->d_tasks = reinterpret_cast<const GemmTask *>(p->dev);
       p->d_units = reinterpret_cast<const WorkUnit *>(static_cast<char *>(p->dev) + o_units);
       p->d_tab = reinterpret_cast<const int64_t *>(static_cast<char *>(p->dev) + o_tab);
+      if (p->nred) {
+        p->d_red = reinterpret_cast<const WorkUnit *>(static_cast<char *>(p->dev) + o_red);
+        p->ws = reinterpret_cast<double *>(static_cast<char *>(p->dev) + o_ws);
+      }
     }
     *pout = p;
     return ITX_OK;
@@ -760,11 +943,20 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   int best = 0;
   const std::string key = tune_key(prec, akc, bkc, maxM, maxN, maxKsum, int(groups.size()));
   const char *force = std::getenv("ITX_FORCE_TILE");
-  if (!st.empty() && maxN <= SK_N && maxK <= SK_K) {
+  // Narrow tasks (all N <= 32, K <= 64) over large C blocks take the
+  // skinny VALU path; C blocks of at most 8192 elements (small QN blocks)
+  // the small-block VALU path; everything else an MFMA tile.
+  double rows = 0;
+  for (auto &g : groups) rows += g.M;
+  const double avg_rows = groups.empty() ? 0.0 : rows / double(groups.size());
+  const bool narrow = maxN <= SK_N && maxK <= SK_K;
+  if (!st.empty() && !force && narrow && avg_rows >= 256) {
     best = kSkinny;
+  } else if (!st.empty() && !force && double(maxM) * double(maxN) <= 8192.0) {
+    best = kSmall;
   } else if (force) {
     best = std::atoi(force);
-    if (best < 0 || best >= kNumTiles || best == kSkinny) best = 0;
+    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow)) best = 0;
   } else if (ctx->tune.count(key)) {
     best = ctx->tune[key];
   } else if (ctx->autotune && !st.empty()) {
@@ -780,7 +972,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     (void)hipEventCreate(&e1);
     float best_ms = 1e30f;
     for (int ti = 0; ti < kNumTiles; ++ti) {
-      if (ti == kSkinny) continue;
+      if (ti == kSkinny || ti == kSmall) continue;
       itx_plan *pc = nullptr;
       if (finalize(ti, &pc)) continue;
       const double *A = static_cast<const double *>(bufA), *B = static_cast<const double *>(bufB);
@@ -807,7 +999,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   } else {
     double best_t = 1e300;
     for (int ti = 0; ti < kNumTiles; ++ti) {
-      if (ti == kSkinny) continue;
+      if (ti == kSkinny || ti == kSmall) continue;
       const TileCfg ts = kTiles[ti];
       std::vector<double> costs;
       for (auto &g : groups) {
@@ -828,14 +1020,26 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   return finalize(best, out);
 }
 
+KernelArgs plan_args(const itx_plan *p, int which, const double *A, const double *B, double *C, double alpha,
+                     double beta) {
+  if (which == 0)  // the contraction (into the workspace when split)
+    return KernelArgs{A, B, p->nred ? p->ws : C, p->d_tasks, p->d_units, p->nunits, p->d_tab, alpha, beta};
+  return KernelArgs{p->ws, nullptr, C, p->d_tasks, p->d_red, p->nred, p->d_tab, alpha, beta};
+}
+
 int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *B, double *C, double alpha,
                 double beta) {
   if (!p) return set_err(ctx, ITX_ERR_ARG, "null plan");
   if (p->nunits == 0) return ITX_OK;
-  KernelArgs ka{A, B, C, p->d_tasks, p->d_units, p->nunits, p->d_tab, alpha, beta};
   void *args[KernelArgs::kCount];
+  KernelArgs ka = plan_args(p, 0, A, B, C, alpha, beta);
   ka.pointers(args);
   ITX_CHECK_HIP(ctx, hipLaunchKernel(p->fn, dim3(p->grid), dim3(p->block), args, 0, ctx->stream));
+  if (p->nred) {
+    KernelArgs kr = plan_args(p, 1, A, B, C, alpha, beta);
+    kr.pointers(args);
+    ITX_CHECK_HIP(ctx, hipLaunchKernel(p->fn_red, dim3(p->grid_red), dim3(SM_T_RED), args, 0, ctx->stream));
+  }
   return ITX_OK;
 }
 
@@ -875,9 +1079,8 @@ int cached_plan(itx_ctx *ctx, const std::string &key, itx_plan **out,
   int rc = make(&p);
   if (rc) return rc;
   if (ctx->plan_cache.size() >= kPlanCacheMax) {
-    // evict the oldest entry (FIFO); its kernels were enqueued before,
-    // so wait for them before the device tables go away
-    hipStreamSynchronize(ctx->stream);
+    // evict the oldest entry (FIFO); its device block returns to the
+    // stream-ordered pool, so kernels already enqueued still read it intact
     const std::string old = ctx->plan_lru.front();
     ctx->plan_lru.erase(ctx->plan_lru.begin());
     auto jt = ctx->plan_cache.find(old);
@@ -923,10 +1126,9 @@ double itx_plan_flops(const itx_plan *plan) { return plan ? plan->flops : 0.0; }
 
 int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan) {
   if (!plan) return ITX_OK;
-  if (plan->dev) {
-    if (ctx) hipStreamSynchronize(ctx->stream);
-    hipFree(plan->dev);
-  }
+  // the device block goes back to the context's stream-ordered pool (no
+  // wait); without a context it stays with the pool until itx_destroy
+  if (plan->dev && ctx) itx::pool_free(ctx, plan->dev);
   delete plan;
   return ITX_OK;
 }
@@ -938,8 +1140,8 @@ int itx_chain_create(itx_ctx *ctx, int nstages, itx_plan *const *plans, itx_chai
     if (!plans[i]) return itx::set_err(ctx, ITX_ERR_ARG, "itx_chain_create: null plan");
   auto *ch = new itx_chain();
   ch->plans.assign(plans, plans + nstages);
-  ch->args.resize(size_t(nstages));
-  ch->nodes.assign(size_t(nstages), nullptr);
+  ch->args.resize(2 * size_t(nstages));
+  ch->nodes.assign(2 * size_t(nstages), nullptr);
   *out = ch;
   return ITX_OK;
 }
@@ -948,49 +1150,54 @@ int itx_chain_launch(itx_ctx *ctx, itx_chain *ch, const double *const *A, const 
                      double *const *C, const double *alpha, const double *beta) {
   if (!ctx || !ch || !A || !B || !C) return ITX_ERR_ARG;
   const int n = int(ch->plans.size());
-  auto args_of = [&](int i) {
-    const itx_plan *p = ch->plans[size_t(i)];
-    return itx::KernelArgs{A[i], B[i], C[i], p->d_tasks, p->d_units, p->nunits, p->d_tab,
-                           alpha ? alpha[i] : 1.0, beta ? beta[i] : 0.0};
+  // node 2i: the contraction of stage i; node 2i+1: its reduction pass
+  auto args_of = [&](int x) {
+    const int i = x / 2;
+    return itx::plan_args(ch->plans[size_t(i)], x % 2, A[i], B[i], C[i], alpha ? alpha[i] : 1.0,
+                          beta ? beta[i] : 0.0);
   };
-  auto params_of = [&](int i, void **argv) {
-    const itx_plan *p = ch->plans[size_t(i)];
+  auto active = [&](int x) {
+    const itx_plan *p = ch->plans[size_t(x / 2)];
+    return p->nunits > 0 && (x % 2 == 0 || p->nred > 0);
+  };
+  auto params_of = [&](int x, void **argv) {
+    const itx_plan *p = ch->plans[size_t(x / 2)];
     hipKernelNodeParams kp{};
-    kp.func = const_cast<void *>(p->fn);
-    kp.gridDim = dim3(unsigned(p->grid));
-    kp.blockDim = dim3(unsigned(p->block));
+    kp.func = const_cast<void *>(x % 2 ? p->fn_red : p->fn);
+    kp.gridDim = dim3(unsigned(x % 2 ? p->grid_red : p->grid));
+    kp.blockDim = dim3(unsigned(x % 2 ? itx::SM_T_RED : p->block));
     kp.sharedMemBytes = 0;
-    ch->args[size_t(i)].pointers(argv);
+    ch->args[size_t(x)].pointers(argv);
     kp.kernelParams = argv;
     kp.extra = nullptr;
     return kp;
   };
   if (!ch->exec) {
-    // first launch: one kernel node per non-empty stage, each depending on
-    // the previous one (stream order of the eager path)
+    // first launch: one kernel node per launch, each after the previous
+    // one (the stream order of the eager path)
     ITX_CHECK_HIP(ctx, hipGraphCreate(&ch->graph, 0));
     hipGraphNode_t prev = nullptr;
-    for (int i = 0; i < n; ++i) {
-      ch->args[size_t(i)] = args_of(i);
-      if (ch->plans[size_t(i)]->nunits == 0) continue;
+    for (int x = 0; x < 2 * n; ++x) {
+      ch->args[size_t(x)] = args_of(x);
+      if (!active(x)) continue;
       void *argv[itx::KernelArgs::kCount];
-      hipKernelNodeParams kp = params_of(i, argv);
-      ITX_CHECK_HIP(ctx, hipGraphAddKernelNode(&ch->nodes[size_t(i)], ch->graph, prev ? &prev : nullptr,
+      hipKernelNodeParams kp = params_of(x, argv);
+      ITX_CHECK_HIP(ctx, hipGraphAddKernelNode(&ch->nodes[size_t(x)], ch->graph, prev ? &prev : nullptr,
                                                prev ? 1 : 0, &kp));
-      prev = ch->nodes[size_t(i)];
+      prev = ch->nodes[size_t(x)];
     }
     ITX_CHECK_HIP(ctx, hipGraphInstantiate(&ch->exec, ch->graph, nullptr, nullptr, 0));
   } else {
     // replay: patch the nodes whose operands or scalars changed
-    for (int i = 0; i < n; ++i) {
-      if (!ch->nodes[size_t(i)]) continue;
-      const itx::KernelArgs a = args_of(i);
-      itx::KernelArgs &c = ch->args[size_t(i)];
+    for (int x = 0; x < 2 * n; ++x) {
+      if (!ch->nodes[size_t(x)]) continue;
+      const itx::KernelArgs a = args_of(x);
+      itx::KernelArgs &c = ch->args[size_t(x)];
       if (a.A == c.A && a.B == c.B && a.C == c.C && a.alpha == c.alpha && a.beta == c.beta) continue;
       c = a;
       void *argv[itx::KernelArgs::kCount];
-      hipKernelNodeParams kp = params_of(i, argv);
-      ITX_CHECK_HIP(ctx, hipGraphExecKernelNodeSetParams(ch->exec, ch->nodes[size_t(i)], &kp));
+      hipKernelNodeParams kp = params_of(x, argv);
+      ITX_CHECK_HIP(ctx, hipGraphExecKernelNodeSetParams(ch->exec, ch->nodes[size_t(x)], &kp));
     }
   }
   ITX_CHECK_HIP(ctx, hipGraphLaunch(ch->exec, ctx->stream));

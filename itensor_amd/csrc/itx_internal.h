@@ -25,9 +25,12 @@ struct GemmTask {
 };
 
 // One output tile of one C block: the tasks [tbeg,tend) all write the
-// same C block and are summed into the tile in registers.
+// same C block and are summed into the tile in registers.  With slot >= 0
+// the unit computes a partial tile (split over the task list) into slot
+// `slot` of the plan's workspace, and a reduction pass writes C.
 struct WorkUnit {
   int32_t tbeg, tend, m0, n0;
+  int32_t slot, pad;
 };
 
 // The argument list shared by every contraction kernel (k_gemm variants
@@ -50,6 +53,7 @@ struct KernelArgs {
 }  // namespace itx
 
 struct itx_plan {
+  itx_ctx *ctx = nullptr;  // owner of the device block (its caching pool)
   int prec = ITX_PREC_F64;
   int tile = 0;        // index into the tile-shape table of the engine
   bool a_kcontig = false, b_kcontig = false;
@@ -63,13 +67,21 @@ struct itx_plan {
   const int64_t *d_tab = nullptr;
   const void *fn = nullptr;  // kernel entry point and workgroup size
   int block = 0;
+  // split plans: partial tiles into ws, then a reduction pass over
+  // nred reduction units (d_red) writes C
+  int nred = 0, grid_red = 0;
+  const itx::WorkUnit *d_red = nullptr;
+  const void *fn_red = nullptr;
+  double *ws = nullptr;
 };
 
-// A fixed sequence of plan executions replayed as one HIP graph.
+// A fixed sequence of plan executions replayed as one HIP graph (one or
+// two kernel nodes per plan: the contraction and, for split plans, its
+// reduction pass).
 struct itx_chain {
   std::vector<const itx_plan *> plans;
   std::vector<itx::KernelArgs> args;  // arguments the graph nodes hold now
-  std::vector<hipGraphNode_t> nodes;  // per stage; null for empty plans
+  std::vector<hipGraphNode_t> nodes;  // 2 per stage; null where no launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -85,6 +97,11 @@ struct itx_ctx {
   size_t cached_bytes = 0;
   // reduction scratch (partials + results)
   double *red = nullptr;
+  // pinned host memory: reduction results written by the device
+  // (hres, device view hres_dev) and the staging arena of uploads
+  double *hres = nullptr, *hres_dev = nullptr;
+  char *stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
   // plan cache for the one-shot contraction entry points
   std::unordered_map<std::string, itx_plan *> plan_cache;
   std::vector<std::string> plan_lru;
@@ -101,6 +118,12 @@ int hip_err(itx_ctx *ctx, hipError_t e, const char *where);
 // Device allocation through the context's caching pool.
 int pool_alloc(itx_ctx *ctx, size_t bytes, void **p);
 int pool_free(itx_ctx *ctx, void *p);
+// Host-to-device copy ordered on the context's stream that returns without
+// waiting: the bytes go through the pinned staging arena (the stream is
+// synchronised only when the arena wraps around; copies larger than half
+// the arena are synchronous).
+int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
+constexpr int HRES_SLOTS = 64;  // pinned result slots (multi-dot up to 16)
 
 // Build a contraction plan (dense = one task).  Shared by the dense and
 // QN entry points.  blk*: per-task block dims (ntasks x r).
@@ -110,6 +133,11 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
                int prec, itx_plan **out);
 int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *B, double *C,
                 double alpha, double beta);
+// Arguments of launch `which` of a plan (0: the contraction, 1: the
+// reduction pass of a split plan) for operands A, B, C.
+KernelArgs plan_args(const itx_plan *p, int which, const double *A, const double *B, double *C, double alpha,
+                     double beta);
+constexpr int SM_T_RED = 256;  // workgroup size of the reduction pass
 
 constexpr int RED_SLOTS = 2048;  // partial-sum slots in ctx->red
 

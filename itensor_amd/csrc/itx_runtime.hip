@@ -65,9 +65,30 @@ int pool_free(itx_ctx *ctx, void *p) {
   return ITX_OK;
 }
 
+int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return ITX_OK;
+  if (!ctx->stage || bytes > ctx->stage_cap / 2) {
+    ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ITX_OK;
+  }
+  size_t off = (ctx->stage_off + 255) & ~size_t(255);
+  if (off + bytes > ctx->stage_cap) {
+    // every earlier copy out of the arena is complete once the stream is
+    ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    off = 0;
+  }
+  std::memcpy(ctx->stage + off, src, bytes);
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, ctx->stage + off, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ctx->stage_off = off + bytes;
+  return ITX_OK;
+}
+
 }  // namespace itx
 
 namespace {
+constexpr size_t kStageBytes = size_t(8) << 20;
+
 __global__ void k_fill(double *x, size_t n, double v) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t stride = size_t(gridDim.x) * blockDim.x;
@@ -112,6 +133,21 @@ int itx_create(int device, itx_ctx **out) {
     delete ctx;
     return ITX_ERR_NOMEM;
   }
+  void *hp = nullptr;
+  e = hipHostMalloc(&hp, sizeof(double) * itx::HRES_SLOTS, hipHostMallocDefault);
+  void *hd = nullptr;
+  if (e == hipSuccess) e = hipHostGetDevicePointer(&hd, hp, 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->stage), kStageBytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+    if (hp) hipHostFree(hp);
+    hipFree(ctx->red);
+    delete ctx;
+    return ITX_ERR_NOMEM;
+  }
+  ctx->hres = static_cast<double *>(hp);
+  ctx->hres_dev = static_cast<double *>(hd);
+  ctx->stage_cap = kStageBytes;
   // ITX_AUTOTUNE=1 turns the tile autotuner on; ITX_TUNE_FILE names a
   // tuning table to load (see itx_tune_load)
   if (const char *a = std::getenv("ITX_AUTOTUNE")) ctx->autotune = std::atoi(a) ? 1 : 0;
@@ -128,12 +164,16 @@ int itx_destroy(itx_ctx *ctx) {
   for (auto &kv : ctx->free_blocks) hipFree(kv.second);
   for (auto &kv : ctx->live) hipFree(kv.first);
   hipFree(ctx->red);
+  if (ctx->hres) hipHostFree(ctx->hres);
+  if (ctx->stage) hipHostFree(ctx->stage);
   delete ctx;
   return ITX_OK;
 }
 
 int itx_set_stream(itx_ctx *ctx, void *stream) {
   if (!ctx) return ITX_ERR_ARG;
+  // uploads and frees of the pool are ordered on the old stream: drain it
+  if (reinterpret_cast<hipStream_t>(stream) != ctx->stream) ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->stream = reinterpret_cast<hipStream_t>(stream);
   return ITX_OK;
 }

@@ -160,9 +160,11 @@ int multi_dot_dev(itx_ctx *ctx, const double *const *V, int nv, const double *y,
   return ITX_OK;
 }
 
-int to_host(itx_ctx *ctx, const double *dsrc, double *hdst, int count) {
-  ITX_CHECK_HIP(ctx, hipMemcpyAsync(hdst, dsrc, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+// reduction results land in the context's pinned result slots; one
+// stream synchronisation makes them visible to the host
+int to_host(itx_ctx *ctx, double *hdst, int count) {
   ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int j = 0; j < count; ++j) hdst[j] = ctx->hres[j];
   return ITX_OK;
 }
 
@@ -179,28 +181,25 @@ int itx_dot_dev(itx_ctx *ctx, const double *x, const double *y, size_t n, double
 
 int itx_dot(itx_ctx *ctx, const double *x, const double *y, size_t n, double *result) {
   if (!ctx || !result) return ITX_ERR_ARG;
-  double *slot = ctx->red + itx::RED_SLOTS;
-  int rc = itx_dot_dev(ctx, x, y, n, slot);
+  int rc = itx_dot_dev(ctx, x, y, n, ctx->hres_dev);
   if (rc) return rc;
-  return itx::to_host(ctx, slot, result, 1);
+  return itx::to_host(ctx, result, 1);
 }
 
 int itx_nrm2(itx_ctx *ctx, const double *x, size_t n, double *result) {
   if (!ctx || !result) return ITX_ERR_ARG;
-  double *slot = ctx->red + itx::RED_SLOTS;
   const double *V[1] = {x};
-  int rc = itx::multi_dot_dev(ctx, V, 1, x, n, slot, 1);
+  int rc = itx::multi_dot_dev(ctx, V, 1, x, n, ctx->hres_dev, 1);
   if (rc) return rc;
-  return itx::to_host(ctx, slot, result, 1);
+  return itx::to_host(ctx, result, 1);
 }
 
 int itx_multi_dot(itx_ctx *ctx, const double *const *V, int nv, const double *y, size_t n, double *out) {
   if (!ctx || !out || nv < 0 || nv > 16) return ITX_ERR_ARG;
   if (nv == 0) return ITX_OK;
-  double *slot = ctx->red + itx::RED_SLOTS;
-  int rc = itx::multi_dot_dev(ctx, V, nv, y, n, slot, 0);
+  int rc = itx::multi_dot_dev(ctx, V, nv, y, n, ctx->hres_dev, 0);
   if (rc) return rc;
-  return itx::to_host(ctx, slot, out, nv);
+  return itx::to_host(ctx, out, nv);
 }
 
 int itx_axpy(itx_ctx *ctx, double a, const double *x, double *y, size_t n) {
@@ -294,16 +293,16 @@ extern "C" int itx_block_copy(itx_ctx *ctx, const double *X, double *Y, int ntas
   std::vector<int64_t> host(per * ntasks + ntasks);
   std::memcpy(host.data(), desc, dbytes);
   std::memcpy(host.data() + per * ntasks, start.data(), sbytes);
-  hipError_t e = hipMemcpyAsync(buf, host.data(), dbytes + sbytes, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) {
-    const int64_t *dd = static_cast<const int64_t *>(buf);
-    hipLaunchKernelGGL(itx::k_block_copy, dim3(unsigned(chunks)), dim3(itx::VT), 0, ctx->stream, X, Y, dd,
-                       dd + per * ntasks, ntasks, r, alpha, beta);
-    e = hipGetLastError();
+  rc = itx::stage_upload(ctx, buf, host.data(), dbytes + sbytes);
+  if (rc) {
+    itx::pool_free(ctx, buf);
+    return rc;
   }
-  // the staging vector must outlive the (possibly asynchronous) upload
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  itx::pool_free(ctx, buf);
+  const int64_t *dd = static_cast<const int64_t *>(buf);
+  hipLaunchKernelGGL(itx::k_block_copy, dim3(unsigned(chunks)), dim3(itx::VT), 0, ctx->stream, X, Y, dd,
+                     dd + per * ntasks, ntasks, r, alpha, beta);
+  hipError_t e = hipGetLastError();
+  itx::pool_free(ctx, buf);  // stream-ordered: reused only by later work
   if (e != hipSuccess) return itx::hip_err(ctx, e, "itx_block_copy");
   return ITX_OK;
 }

@@ -551,8 +551,17 @@ ITensor &ITensor::operator*=(ITensor const &R) {
 ITensor &ITensor::operator*=(Real r) {
   if (!*this) throw ITError("scaling a default constructed ITensor");
   if (isCombiner()) throw ITError("scaling a combiner");
-  detach();
-  if (store_->size) check(itx_scal(ctx(), r, store_->data->p, store_->size), "itx_scal");
+  if (!store_->size) return *this;
+  if (store_.use_count() > 1) {
+    // shared storage (copy-on-write): write the scaled copy directly
+    auto s = std::make_shared<Storage>(*store_);
+    s->data = devAlloc(store_->size);
+    const double *V[1] = {store_->data->p};
+    check(itx_lincomb(ctx(), V, &r, 1, s->data->p, store_->size), "itx_lincomb");
+    store_ = s;
+    return *this;
+  }
+  check(itx_scal(ctx(), r, store_->data->p, store_->size), "itx_scal");
   return *this;
 }
 
@@ -591,6 +600,14 @@ void daxpy(ITensor &L, ITensor const &R, Real alpha) {
   }
   checkArrows(L.inds(), R.inds(), true);
   if (!L.isQN()) {
+    if (ident && L.storePtr().use_count() > 1) {
+      auto st = mkDense(L.store().size);
+      const double *V[2] = {L.store().data->p, R.store().data->p};
+      const double c[2] = {1.0, alpha};
+      if (st->size) check(itx_lincomb(ctx(), V, c, 2, st->data->p, st->size), "itx_lincomb");
+      L = ITensor(L.inds(), st);
+      return;
+    }
     L.detach();
     auto &Ls = L.store();
     if (ident) {
@@ -607,7 +624,15 @@ void daxpy(ITensor &L, ITensor const &R, Real alpha) {
     return;
   }
   if (ident && sameOffsets(L.store().offsets, R.store().offsets)) {
-    L.detach();
+    if (L.storePtr().use_count() > 1) {
+      // shared storage: one kernel writes L + alpha R into a new block
+      auto st = mkQDense(L.store().offsets, L.store().size, false);
+      const double *V[2] = {L.store().data->p, R.store().data->p};
+      const double c[2] = {1.0, alpha};
+      if (st->size) check(itx_lincomb(ctx(), V, c, 2, st->data->p, st->size), "itx_lincomb");
+      L = ITensor(L.inds(), st);
+      return;
+    }
     check(itx_axpy(ctx(), alpha, R.store().data->p, L.store().data->p, L.store().size), "itx_axpy");
     return;
   }
@@ -914,6 +939,52 @@ void randomize(ITensor &T, uint64_t seed) {
   }
   if (!h.empty())
     check(itx_memcpy_htod(ctx(), T.store().data->p, h.data(), sizeof(double) * h.size()), "itx_memcpy_htod");
+}
+
+// ---------------------------------------------- batched vector helpers --
+// Same index order (with arrows) and, for QDense, the same block list:
+// the storages are then aligned element by element.
+static bool alignedStorage(ITensor const &a, ITensor const &b) {
+  if (!a || !b || a.isCombiner() || b.isCombiner() || a.isQN() != b.isQN()) return false;
+  if (a.order() != b.order() || a.store().size != b.store().size) return false;
+  for (int i = 0; i < a.order(); ++i) {
+    Index const &x = a.inds()[size_t(i)], &y = b.inds()[size_t(i)];
+    if (!(x == y) || x.dir() != y.dir()) return false;
+  }
+  return !a.isQN() || sameOffsets(a.store().offsets, b.store().offsets);
+}
+
+ITensor lincomb(std::vector<ITensor const *> const &V, std::vector<Real> const &c) {
+  if (V.empty() || V.size() != c.size()) throw ITError("lincomb: empty or mismatched input");
+  bool aligned = V.size() <= 16;
+  for (size_t k = 1; k < V.size() && aligned; ++k) aligned = alignedStorage(*V[0], *V[k]);
+  if (!aligned) {
+    ITensor y = c[0] * (*V[0]);
+    for (size_t k = 1; k < V.size(); ++k) y += c[k] * (*V[k]);
+    return y;
+  }
+  auto st = std::make_shared<Storage>(V[0]->store());
+  st->data = devAlloc(st->size);
+  std::vector<const double *> p(V.size());
+  for (size_t k = 0; k < V.size(); ++k) p[k] = V[k]->store().data->p;
+  if (st->size) check(itx_lincomb(ctx(), p.data(), c.data(), int(V.size()), st->data->p, st->size), "itx_lincomb");
+  return ITensor(V[0]->inds(), st);
+}
+
+std::vector<Real> dotDagMany(std::vector<ITensor const *> const &V, ITensor const &q) {
+  std::vector<Real> out(V.size(), 0.0);
+  bool aligned = V.size() <= 16;
+  for (size_t k = 0; k < V.size() && aligned; ++k) aligned = alignedStorage(*V[k], q);
+  if (!aligned) {
+    for (size_t k = 0; k < V.size(); ++k) out[k] = dotDag(*V[k], q);
+    return out;
+  }
+  if (V.empty() || q.store().size == 0) return out;
+  std::vector<const double *> p(V.size());
+  for (size_t k = 0; k < V.size(); ++k) p[k] = V[k]->store().data->p;
+  check(itx_multi_dot(ctx(), p.data(), int(V.size()), q.store().data->p, q.store().size, out.data()),
+        "itx_multi_dot");
+  return out;
 }
 
 // randomITensor(QN, IndexSet) / randomITensor(IndexSet) (itensor.h:969-975):

@@ -268,6 +268,12 @@ ITensor operator-(ITensor L, ITensor const &R);
 void daxpy(ITensor &L, ITensor const &R, Real alpha);
 
 Real norm(ITensor const &T);
+// sum_k c[k] * V[k] (one kernel when the storages are aligned: same index
+// order and block list; else through +=)
+ITensor lincomb(std::vector<ITensor const *> const &V, std::vector<Real> const &c);
+// eltC(dag(V[k]) * q) for every k (one reduction launch and one wait when
+// the storages are aligned)
+std::vector<Real> dotDagMany(std::vector<ITensor const *> const &V, ITensor const &q);
 // eltC(dag(A)*B) for real tensors
 Real dotDag(ITensor const &A, ITensor const &B);
 ITensor dag(ITensor T);

@@ -62,7 +62,7 @@ Real davidson(BigMatrixT const &A, ITensor &phi, Args const &args, DavidsonStats
     if (ii == 0) {
       lambda = initEn;
       M[0] = lambda;
-      q = AV[0] - lambda * V[0];
+      q = lincomb({&AV[0], &V[0]}, {1.0, -lambda});
     } else {
       const int m = int(ii + 1);
       std::vector<double> Mr(size_t(m * m)), w, U;
@@ -70,18 +70,24 @@ Real davidson(BigMatrixT const &A, ITensor &phi, Args const &args, DavidsonStats
         for (int c = 0; c < m; ++c) Mr[size_t(r * m + c)] = M[size_t(r) * MS + size_t(c)];
       detail::smallSymEig(m, Mr, w, U);
       lambda = w[0];
-      auto Ut = [&](int k) { return U[size_t(k * m + 0)]; };
-      phi_t = Ut(0) * V[0];
-      q = Ut(0) * AV[0];
-      for (int k = 1; k <= int(ii); ++k) {
-        phi_t += Ut(k) * V[size_t(k)];
-        q += Ut(k) * AV[size_t(k)];
+      // phi_t = sum_k U_k0 V_k, q = sum_k U_k0 AV_k - lambda phi_t, both
+      // negated when U_00 < 0 (iterativesolvers.h:257-283)
+      const double sgn = U[0] < 0 ? -1.0 : 1.0;
+      std::vector<ITensor const *> vp, qp;
+      std::vector<double> vc, qc;
+      for (int k = 0; k <= int(ii); ++k) {
+        const double u = sgn * U[size_t(k * m + 0)];
+        vp.push_back(&V[size_t(k)]);
+        vc.push_back(u);
+        qp.push_back(&AV[size_t(k)]);
+        qc.push_back(u);
       }
-      q += (-lambda) * phi_t;
-      if (Ut(0) < 0) {
-        phi_t *= -1;
-        q *= -1;
+      for (int k = 0; k <= int(ii); ++k) {
+        qp.push_back(&V[size_t(k)]);
+        qc.push_back(-lambda * vc[size_t(k)]);
       }
+      phi_t = lincomb(vp, vc);
+      q = lincomb(qp, qc);
     }
     qnorm = norm(q);
     const bool converged = (qnorm < errgoal && std::abs(lambda - last_lambda) < errgoal) ||
@@ -89,14 +95,22 @@ Real davidson(BigMatrixT const &A, ITensor &phi, Args const &args, DavidsonStats
     last_lambda = lambda;
     if (qnorm < 1e-20 || (converged && ii >= miniter) || ii == actual_maxiter) break;
 
-    // one classical Gram-Schmidt pass (iterativesolvers.h:305-357)
+    // one classical Gram-Schmidt pass (iterativesolvers.h:305-357): all
+    // projections from one batched reduction, one linear combination
     int Npass = 1, pass = 1, tot_pass = 0;
     bool stop = false;
     while (pass <= Npass) {
       ++tot_pass;
-      std::vector<double> Vq(ni);
-      for (size_t k = 0; k < ni; ++k) Vq[k] = dotDag(V[k], q);
-      for (size_t k = 0; k < ni; ++k) q += (-Vq[k]) * V[k];
+      std::vector<ITensor const *> vk;
+      for (size_t k = 0; k < ni; ++k) vk.push_back(&V[k]);
+      std::vector<double> Vq = dotDagMany(vk, q);
+      std::vector<ITensor const *> gp{&q};
+      std::vector<double> gc{1.0};
+      for (size_t k = 0; k < ni; ++k) {
+        gp.push_back(&V[k]);
+        gc.push_back(-Vq[k]);
+      }
+      q = lincomb(gp, gc);
       double qnrm = norm(q);
       if (qnrm < 1e-10) {
         q = V[ni - 1];
@@ -115,10 +129,12 @@ Real davidson(BigMatrixT const &A, ITensor &phi, Args const &args, DavidsonStats
     V[ni] = q;
     A.product(V[ni], AV[ni]);
     ++nprod;
+    std::vector<ITensor const *> vk;
+    for (size_t k = 0; k <= ni; ++k) vk.push_back(&V[k]);
+    const std::vector<double> z = dotDagMany(vk, AV[ni]);
     for (size_t k = 0; k <= ni; ++k) {
-      const double z = dotDag(V[k], AV[ni]);
-      M[k * MS + ni] = z;
-      M[ni * MS + k] = z;
+      M[k * MS + ni] = z[k];
+      M[ni * MS + k] = z[k];
     }
     ++iters;
   }

@@ -299,6 +299,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   PH.setPrecision(int(args.getInt("HpsiPrecision", ITX_PREC_F64)));
   PH.setFusion(args.getBool("FuseMPO", true));
   PH.setMode(HpsiMode(args.getInt("HpsiMode", long(HpsiMode::Graph))));
+  const bool syncTiming = args.getBool("SyncTiming", false);
   Real energy = NAN;
   for (size_t sw = 0; sw < sweeps.size(); ++sw) {
     auto const &S = sweeps[sw];
@@ -311,16 +312,25 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
     args.add("MaxIter", S.niter);
     if (S.noise != 0) throw ITError("dmrg: noise term (deltaRho) is not part of this build");
     for (int b = 1, ha = 1; ha <= 2; sweepnext(b, ha, N)) {
-      auto t0 = std::chrono::steady_clock::now();
+      using clk = std::chrono::steady_clock;
+      auto secs = [](clk::time_point a, clk::time_point b2) { return std::chrono::duration<double>(b2 - a).count(); };
+      auto mark = [&]() {
+        if (syncTiming) check(itx_synchronize(ctx()), "itx_synchronize");
+        return clk::now();
+      };
+      auto t0 = clk::now();
       PH.position(b, psi);
+      auto t1 = mark();
       ITensor phi = psi(b) * psi(b + 1);
       PH.fuseOps(phi.inds());
       DavidsonStats st;
       energy = davidson(PH, phi, args, &st);
+      auto t2 = mark();
       auto spec = svdBond(psi, b, phi, ha == 1 ? Fromleft : Fromright, args);
+      auto t3 = mark();
       if (obs) {
         BondRecord r{int(sw + 1), b, ha, energy, spec.truncerr, commonIndex(psi(b), psi(b + 1)).dim(), st.products,
-                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()};
+                     secs(t0, t3), secs(t0, t1), secs(t1, t2), secs(t2, t3)};
         obs(r);
       }
     }

@@ -145,6 +145,9 @@ struct BondRecord {
   long dim;
   int products;
   double seconds;
+  // wall time of the environment update (position), davidson and svdBond;
+  // device work is attributed to its phase only with Args "SyncTiming"
+  double t_env = 0, t_davidson = 0, t_svd = 0;
 };
 
 // MPS::svdBond, density-matrix route (mps/mps_impl.h:21-99)

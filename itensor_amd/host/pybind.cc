@@ -72,6 +72,9 @@ py::dict recordDict(BondRecord const &r) {
   d["dim"] = r.dim;
   d["nprod"] = r.products;
   d["seconds"] = r.seconds;
+  d["t_env"] = r.t_env;
+  d["t_davidson"] = r.t_davidson;
+  d["t_svd"] = r.t_svd;
   return d;
 }
 

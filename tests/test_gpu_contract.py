@@ -207,3 +207,95 @@ def test_chain_graph_replay_matches_direct(ctx, dev):
         chain.close()
         for p in plans:
             p.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_small_block_kernel_general_shapes(ctx, dev, monkeypatch, prec):
+    """The small-block VALU kernel forced onto every case shape (K above one
+    64-wide chunk, N above one 32-wide unit, rank-0 results, permuted
+    operands), with alpha and beta: FP64 arithmetic in every form."""
+    monkeypatch.setenv("ITX_FORCE_TILE", "10")
+    rng = np.random.default_rng(55)
+    for case, (dA, labA, dB, labB, labC) in enumerate(CASES):
+        A = rng.standard_normal(dA)
+        B = rng.standard_normal(dB)
+        ref0 = contract(A, labA, B, labB, labC)
+        C0 = rng.standard_normal(ref0.shape) if ref0.shape else np.array(rng.standard_normal())
+        ref = contract(A, labA, B, labB, labC, alpha=0.7, beta=0.3, C=C0)
+        plan = ctx.plan_dense(A.shape, labA, B.shape, labB, ref.shape, labC, prec)
+        try:
+            d_A, d_B, d_C = dev.put(A), dev.put(B), dev.put(C0)
+            plan.execute(d_A.data_ptr(), d_B.data_ptr(), d_C.data_ptr(), 0.7, 0.3)
+            got = dev.get(d_C, ref.shape)
+        finally:
+            plan.close()
+        assert relerr(got, ref) < TOL64, case
+
+
+SPLIT_CASES = [
+    # few small output blocks, long contracted extent: split over k chunks
+    ((20, 5000), (1, -1), (5000, 20), (-1, 2), (1, 2)),
+    ((3, 700, 7), (1, -1, 2), (700, 5), (-1, 3), (3, 1, 2)),
+    ((3000, 64), (-1, 1), (3000, 33), (-1, 2), (2, 1)),
+    ((40, 40, 9), (-1, -2, 1), (40, 40, 11), (-2, -1, 2), (1, 2)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SPLIT_CASES)))
+def test_split_small_plans(ctx, dev, case):
+    """Small output blocks with a long contracted extent: the plan splits
+    the k chunks over several workgroups and a reduction pass writes C
+    (alpha, beta honoured); the result matches the oracle to 1e-12 and is
+    bitwise reproducible across executions."""
+    dA, labA, dB, labB, labC = SPLIT_CASES[case]
+    rng = np.random.default_rng(300 + case)
+    A = rng.standard_normal(dA)
+    B = rng.standard_normal(dB)
+    C0 = rng.standard_normal(contract(A, labA, B, labB, labC).shape)
+    ref = contract(A, labA, B, labB, labC, alpha=1.3, beta=-0.4, C=C0)
+    plan = ctx.plan_dense(A.shape, labA, B.shape, labB, ref.shape, labC, "f64")
+    try:
+        d_A, d_B = dev.put(A), dev.put(B)
+        outs = []
+        for _ in range(2):
+            d_C = dev.put(C0)
+            plan.execute(d_A.data_ptr(), d_B.data_ptr(), d_C.data_ptr(), 1.3, -0.4)
+            outs.append(dev.get(d_C, ref.shape))
+    finally:
+        plan.close()
+    assert relerr(outs[0], ref) < TOL64
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_chain_with_split_plan(ctx, dev):
+    """A two-stage chain whose first plan is split (contraction + reduction
+    nodes) replayed with new operands matches direct execution bitwise."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((20, 4000))
+    B = rng.standard_normal((4000, 20))
+    D = rng.standard_normal((20, 30))
+    p1 = ctx.plan_dense((20, 4000), (1, -1), (4000, 20), (-1, 2), (20, 20), (1, 2))
+    p2 = ctx.plan_dense((20, 20), (1, -1), (20, 30), (-1, 2), (20, 30), (1, 2))
+    chain = ctx.chain([p1, p2])
+    try:
+        dB, dD = dev.put(B), dev.put(D)
+        mid = dev.empty(400)
+        for rep in range(3):
+            Ar = A * (rep + 1)
+            dA = dev.put(Ar)
+            out = dev.empty(600)
+            chain.launch([dA.data_ptr(), mid.data_ptr()], [dB.data_ptr(), dD.data_ptr()],
+                         [mid.data_ptr(), out.data_ptr()])
+            torch.cuda.synchronize()
+            ref_out = dev.empty(600)
+            p1.execute(dA.data_ptr(), dB.data_ptr(), mid.data_ptr())
+            p2.execute(mid.data_ptr(), dD.data_ptr(), ref_out.data_ptr())
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref_out), rep
+            assert relerr(dev.get(out, (20, 30)), (Ar @ B) @ D) < TOL64
+    finally:
+        chain.close()
+        p1.close()
+        p2.close()

@@ -55,11 +55,14 @@ def test_permute_add(ctx, dev, dims, perm):
     assert relerr(dev.get(dY, dims), np.transpose(X, perm)) == 0.0
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 10, 64, 129, 400])
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 64, 65, 100, 128, 129, 400])
 def test_syev(ctx, dev, n):
+    """Single matrices of every size class (LDS-resident up to 64 and up to
+    128, global above): eigenvalues vs numpy, orthonormal vectors,
+    reconstruction, descending order."""
     rng = np.random.default_rng(1000 + n)
     X = rng.standard_normal((n, n))
-    if n == 64:
+    if n in (64, 128):
         # density-matrix-like: PSD with exact zero eigenvalues and degeneracies
         X = X[:, :20] @ X[:, :20].T
         X[:, :2] = 0
@@ -77,3 +80,41 @@ def test_syev(ctx, dev, n):
     assert np.all(np.diff(w) <= 0)  # descending
     assert np.max(np.abs(U.T @ U - np.eye(n))) < 1e-12
     assert np.max(np.abs(U @ np.diag(w) @ U.T - X)) / scale < 1e-12
+
+
+def test_syev_batched_mixed_classes(ctx, dev):
+    """One batched call over matrices of all three size classes, including
+    degenerate and rank-deficient ones, stored back to back."""
+    rng = np.random.default_rng(77)
+    ns = [5, 64, 1, 90, 200, 33, 128, 0, 150]
+    mats = []
+    for k, n in enumerate(ns):
+        X = rng.standard_normal((n, n))
+        if k % 3 == 1 and n > 4:
+            Q, _ = np.linalg.qr(X)
+            ev = np.repeat(rng.standard_normal((n + 2) // 3), 3)[:n]
+            X = (Q * ev) @ Q.T
+        elif k % 3 == 2 and n > 4:
+            Y = X[:, : n // 4]
+            X = Y @ Y.T
+        else:
+            X = X + X.T
+        mats.append(X)
+    a_offs = np.cumsum([0] + [n * n for n in ns])[:-1]
+    w_offs = np.cumsum([0] + ns)[:-1]
+    flat = np.concatenate([m.ravel(order="F") for m in mats] + [np.zeros(1)])
+    dA = dev.put(flat)
+    dw = dev.empty(sum(ns) + 1)
+    ctx.syev_batched(ns, a_offs, dA.data_ptr(), w_offs, dw.data_ptr())
+    A = dev.get(dA, (flat.size,))
+    w = dev.get(dw, (sum(ns) + 1,))
+    for X, n, ao, wo in zip(mats, ns, a_offs, w_offs):
+        if n == 0:
+            continue
+        U = A[ao: ao + n * n].reshape((n, n), order="F")
+        wn = w[wo: wo + n]
+        wref = np.linalg.eigvalsh(X)[::-1]
+        scale = max(1.0, np.max(np.abs(wref)))
+        assert np.max(np.abs(wn - wref)) / scale < 1e-12, n
+        assert np.max(np.abs(U.T @ U - np.eye(n))) < 1e-12, n
+        assert np.max(np.abs(U @ np.diag(wn) @ U.T - X)) / scale < 1e-12, n
