@@ -21,7 +21,8 @@ namespace itx {
 namespace {
 
 constexpr int ET = 512;       // threads per matrix
-constexpr int EMAX = 2048;    // largest n held in LDS (d, e, rotations)
+constexpr int EMAX = 2048;    // largest n (d, e held in LDS)
+constexpr int ROTG = 6144;    // rotation buffer of the global-memory kernel
 constexpr int QL_MAXIT = 60;  // QL sweeps per eigenvalue before giving up
 
 struct EigTask {
@@ -47,9 +48,11 @@ __device__ __forceinline__ void givens(double f, double g, double &r, double &s,
   c = g * ri;
 }
 
-// One implicit QL sweep of tql2 on (d, e)[l..m] (thread 0 only); the plane
-// rotations are recorded in (rc, rs, ri) for the eigenvector update.
-// Returns the number of rotations.
+// One implicit QL sweep of tql2 on (d, e)[l..m].  Executed by all lanes of
+// wave 0 with identical values (uniform control flow; LDS stores of equal
+// values); the plane rotations are appended to (rc, rs, ri).  Operands of
+// the next step are loaded before the current step's dependency chain.
+// Returns the number of rotations (negative - 1 on an early split).
 __device__ int ql_sweep(double *d, double *e, int l, int m, double *rc, double *rs, int *ri) {
   double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
   double r = sqrt(fma(g, g, 1.0));
@@ -58,8 +61,8 @@ __device__ int ql_sweep(double *d, double *e, int l, int m, double *rc, double *
   int nrot = 0;
   double ei = e[m - 1], di = d[m - 1], di1 = d[m];
   for (int i = m - 1; i >= l; --i) {
-    // operands of the next step are loaded before this one's chain
-    const double ei_next = i > l ? e[i - 1] : 0.0, di_next = i > l ? d[i - 1] : 0.0;
+    const int in = i > 0 ? i - 1 : 0;
+    const double ei_next = e[in], di_next = d[in];
     const double f = s * ei, b = c * ei;
     double cs, sn;
     givens(f, g, r, sn, cs);
@@ -67,7 +70,7 @@ __device__ int ql_sweep(double *d, double *e, int l, int m, double *rc, double *
     if (r == 0) {
       d[i + 1] = di1 - p;
       e[m] = 0;
-      return -(nrot + 1);  // early deflation: d[l], e[l] stay
+      return -(nrot + 1);  // early split: d[l], e[l] stay
     }
     s = sn;
     c = cs;
@@ -90,6 +93,87 @@ __device__ int ql_sweep(double *d, double *e, int l, int m, double *rc, double *
   return nrot;
 }
 
+// Implicit QL on the tridiagonal (d, e) with the deflation test of tql2
+// (|e_m| against the largest |d|+|e| seen so far, so eigenvalues near zero
+// converge to absolute accuracy), rotations applied to the columns of Z
+// (element (r, c) at Z[r * zr + c * zc]).  Wave 0 runs the scalar
+// recurrence -- it never depends on Z -- and fills the rotation buffer
+// (cap entries) sweep by sweep; then every thread applies the buffered
+// rotations to its rows of Z, carrying the updated element of one
+// rotation into the next one in a register.  Returns 1 when an eigenvalue
+// needs more than QL_MAXIT sweeps.
+template <int NT>
+__device__ int ql_phase(int n, double *d, double *e, double *rc, double *rs, int *ri, int cap, int *sh,
+                        double *Z, int zr, int zc) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave 0's recurrence state (uniform across its lanes)
+  int l = 0, iter = 0;
+  bool newl = true;
+  double tst1 = 0;
+  while (true) {
+    if (wave == 0) {
+      int cnt = 0, done = 0, fail = 0;
+      while (cnt + n <= cap) {
+        if (l >= n) {
+          done = 1;
+          break;
+        }
+        if (newl) {
+          tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
+          iter = 0;
+          newl = false;
+        }
+        int m = n - 1;
+        for (int base = l; base < n - 1; base += 64) {
+          const int mm = base + lane;
+          const bool neg = mm < n - 1 && fabs(e[mm]) <= DBL_EPSILON * fmax(fabs(d[mm]) + fabs(d[mm + 1]), tst1);
+          const unsigned long long bal = __ballot(neg);
+          if (bal) {
+            m = base + __ffsll(bal) - 1;
+            break;
+          }
+        }
+        if (m == l) {
+          ++l;
+          newl = true;
+          continue;
+        }
+        if (iter >= QL_MAXIT) {
+          fail = 1;
+          break;
+        }
+        const int nr = ql_sweep(d, e, l, m, rc + cnt, rs + cnt, ri + cnt);
+        cnt += nr < 0 ? -nr - 1 : nr;
+        ++iter;
+      }
+      if (lane == 0) {
+        sh[0] = cnt;
+        sh[1] = done;
+        sh[2] = fail;
+      }
+    }
+    __syncthreads();
+    const int cnt = sh[0], done = sh[1], fail = sh[2];
+    for (int r = tid; r < n; r += NT) {
+      double *zrow = Z + size_t(r) * zr;
+      int prev = -2;
+      double carry = 0;
+      for (int q = 0; q < cnt; ++q) {
+        const int i = ri[q];
+        const double c = rc[q], sn = rs[q];
+        const double f = (i + 1 == prev) ? carry : zrow[size_t(i + 1) * zc];
+        const double z = zrow[size_t(i) * zc];
+        zrow[size_t(i + 1) * zc] = sn * z + c * f;
+        carry = c * z - sn * f;
+        zrow[size_t(i) * zc] = carry;
+        prev = i;
+      }
+    }
+    __syncthreads();  // the buffer and sh are read before wave 0 rewrites them
+    if (done || fail) return fail;
+  }
+}
+
 __device__ __forceinline__ double bsum(double v, double *red) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -106,10 +190,10 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
                                              double *__restrict__ Wbase, double *__restrict__ Zbase,
                                              int *__restrict__ status) {
   __shared__ double d[EMAX], e[EMAX];
-  __shared__ double rc[EMAX], rs[EMAX];
-  __shared__ int ri[EMAX];
+  __shared__ double rc[ROTG], rs[ROTG];
+  __shared__ int ri[ROTG];
   __shared__ double red[ET / 64];
-  __shared__ int sh_m, sh_nrot, sh_fail;
+  __shared__ int sh[4];
 
   const EigTask tk = tasks[blockIdx.x];
   const int n = tk.n;
@@ -170,7 +254,6 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
   if (tid == 0) {
     if (n >= 2) e[n - 2] = A[size_t(n - 2) * n + (n - 1)];
     e[n - 1] = 0;
-    sh_fail = 0;
   }
   __syncthreads();
 
@@ -193,52 +276,7 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
   }
 
   // ---- 3. implicit QL on (d, e), rotations applied to Z's columns ----
-  // Deflation test of tql2: |e_m| small against the largest |d|+|e| seen so
-  // far (tst1), not only against its neighbours, so eigenvalues near zero
-  // (rank-deficient density matrices) converge to absolute accuracy.
-  int failed = 0;
-  double tst1 = 0;  // used by thread 0 only
-  for (int l = 0; l < n && !failed; ++l) {
-    int iter = 0;
-    if (tid == 0) tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
-    while (true) {
-      if (tid == 0) {
-        int m = l;
-        for (; m < n - 1; ++m) {
-          const double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= DBL_EPSILON * fmax(dd, tst1)) break;
-        }
-        sh_m = m;
-        sh_nrot = 0;
-        if (m != l) {
-          if (iter >= QL_MAXIT) {
-            sh_fail = 1;
-          } else {
-            const int nr = ql_sweep(d, e, l, m, rc, rs, ri);
-            sh_nrot = nr < 0 ? -nr - 1 : nr;
-          }
-        }
-      }
-      __syncthreads();
-      const int m = sh_m, nrot = sh_nrot, fail = sh_fail;
-      __syncthreads();  // every thread has read sh_* before thread 0 rewrites them
-      failed = fail;
-      if (m == l || fail) break;
-      for (int r = tid; r < n; r += ET) {
-        double *Zr = Z + size_t(r) * n;
-        for (int q = 0; q < nrot; ++q) {
-          const int i = ri[q];
-          const double f = Zr[i + 1], z = Zr[i];
-          Zr[i + 1] = rs[q] * z + rc[q] * f;
-          Zr[i] = rc[q] * z - rs[q] * f;
-        }
-      }
-      __syncthreads();
-      ++iter;
-    }
-  }
-  __syncthreads();
-  if (sh_fail) {
+  if (ql_phase<ET>(n, d, e, rc, rs, ri, ROTG, sh, Z, n, 1)) {
     if (tid == 0) status[blockIdx.x] = 1;
     return;
   }
@@ -278,15 +316,17 @@ __device__ __forceinline__ double bsum_s(double v, double *red) {
   return s;
 }
 
-template <int NMAX>
+template <int NMAX, int ROT>
 __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
                                                  double *__restrict__ Wbase, int *__restrict__ status) {
+  static_assert(ROT >= 2 * NMAX, "rotation buffer below two sweeps");
   constexpr int LD = NMAX + 1;
   __shared__ double S[NMAX * LD];
-  __shared__ double d[NMAX], e[NMAX], rc[NMAX], rs[NMAX], tau[NMAX], pv[NMAX];
-  __shared__ int ri[NMAX];
+  __shared__ double d[NMAX], e[NMAX], tau[NMAX], pv[NMAX];
+  __shared__ double rc[ROT], rs[ROT];
+  __shared__ int ri[ROT];
   __shared__ double red[ES / 64];
-  __shared__ int sh_m, sh_nrot, sh_fail;
+  __shared__ int sh[4];
 
   const EigTask tk = tasks[blockIdx.x];
   const int n = tk.n;
@@ -348,7 +388,6 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
   if (tid == 0) {
     if (n >= 2) e[n - 2] = S[(n - 2) * LD + (n - 1)];
     e[n - 1] = 0;
-    sh_fail = 0;
     S[(n - 1) * LD + (n - 1)] = 1.0;
   }
   __syncthreads();
@@ -382,48 +421,7 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
   __syncthreads();
 
   // ---- 3. implicit QL (tql2 deflation), rotations on Q's columns ----
-  int failed = 0;
-  double tst1 = 0;
-  for (int l = 0; l < n && !failed; ++l) {
-    int iter = 0;
-    if (tid == 0) tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
-    while (true) {
-      if (tid == 0) {
-        int m = l;
-        for (; m < n - 1; ++m) {
-          const double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= DBL_EPSILON * fmax(dd, tst1)) break;
-        }
-        sh_m = m;
-        sh_nrot = 0;
-        if (m != l) {
-          if (iter >= QL_MAXIT) {
-            sh_fail = 1;
-          } else {
-            const int nr = ql_sweep(d, e, l, m, rc, rs, ri);
-            sh_nrot = nr < 0 ? -nr - 1 : nr;
-          }
-        }
-      }
-      __syncthreads();
-      const int m = sh_m, nrot = sh_nrot, fail = sh_fail;
-      __syncthreads();
-      failed = fail;
-      if (m == l || fail) break;
-      for (int r = tid; r < n; r += ES) {
-        for (int q = 0; q < nrot; ++q) {
-          const int i = ri[q];
-          const double f = S[(i + 1) * LD + r], z = S[i * LD + r];
-          S[(i + 1) * LD + r] = rs[q] * z + rc[q] * f;
-          S[i * LD + r] = rc[q] * z - rs[q] * f;
-        }
-      }
-      __syncthreads();
-      ++iter;
-    }
-  }
-  __syncthreads();
-  if (sh_fail) {
+  if (ql_phase<ES>(n, d, e, rc, rs, ri, ROT, sh, S, 1, LD)) {
     if (tid == 0) status[blockIdx.x] = 1;
     return;
   }
@@ -492,9 +490,9 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   int *dst = reinterpret_cast<int *>(wsb + tbytes);
   const int n64 = int(t64.size()), n128 = int(t128.size()), ng = int(tg.size());
   if (n64)
-    hipLaunchKernelGGL(k_syev_lds<64>, dim3(n64), dim3(ES), 0, ctx->stream, dt, Abase, Wbase, dst);
+    hipLaunchKernelGGL((k_syev_lds<64, 4096>), dim3(n64), dim3(ES), 0, ctx->stream, dt, Abase, Wbase, dst);
   if (n128)
-    hipLaunchKernelGGL(k_syev_lds<128>, dim3(n128), dim3(ES), 0, ctx->stream, dt + n64, Abase, Wbase, dst + n64);
+    hipLaunchKernelGGL((k_syev_lds<128, 1024>), dim3(n128), dim3(ES), 0, ctx->stream, dt + n64, Abase, Wbase, dst + n64);
   if (ng)
     hipLaunchKernelGGL(k_syev, dim3(ng), dim3(ET), 0, ctx->stream, dt + n64 + n128, Abase, Wbase,
                        reinterpret_cast<double *>(wsb + zoff), dst + n64 + n128);
