@@ -214,6 +214,21 @@ int itx_syev(itx_ctx *ctx, int n, double *A, double *w);
 int itx_syev_batched(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
                      const int64_t *w_offs, double *w);
 
+/* ---- singular value decomposition (FP64) --------------------------------
+ * A (m x n, column-major, device, not modified) = U diag(S) V^T with
+ * k = min(m,n): U is m x k and V is n x k (column-major, orthonormal
+ * columns), S holds the k singular values in DESCENDING order.  Columns
+ * of the long side that belong to an exactly zero singular value are set
+ * to zero.  Replaces SVD(M,U,D,V) as svdImpl calls it per matrix block
+ * (svd.cc:87,206-213 -> SVDRef, tensor/algs.cc:586-631). */
+int itx_gesvd(itx_ctx *ctx, int m, int n, const double *A, double *U, double *S, double *V);
+/* nmat independent problems in one launch (the blocks of a QN tensor):
+ * matrix b (m[b] x n[b]) at A + a_offs[b], factors at U + u_offs[b],
+ * S + s_offs[b], V + v_offs[b]. */
+int itx_gesvd_batched(itx_ctx *ctx, int nmat, const int32_t *m, const int32_t *n, const int64_t *a_offs,
+                      const double *A, const int64_t *u_offs, double *U, const int64_t *s_offs, double *S,
+                      const int64_t *v_offs, double *V);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ CSRC = os.path.join(PKG, "csrc")
 HOST = os.path.join(PKG, "host")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["itx_runtime.hip", "itx_contract.hip", "itx_vec.hip", "itx_eig.hip"]
+HIP_SOURCES = ["itx_runtime.hip", "itx_contract.hip", "itx_vec.hip", "itx_eig.hip", "itx_svd.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (they have
 # produced wrong results on this hardware); -O3 for the kernels.

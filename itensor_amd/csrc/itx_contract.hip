@@ -270,6 +270,215 @@ __global__ __launch_bounds__(64 * GM * GN) void k_gemm(const double *__restrict_
   }
 }
 
+// ---- BF16x3 form ----------------------------------------------------------
+// Each FP64 operand value is rounded to FP32 and split into two BF16 terms
+// x ~ hi + lo (hi = bf16(x), lo = bf16(x - hi): 16-17 significant bits);
+// a product is formed from the three leading terms hi*hi + hi*lo + lo*hi on
+// the BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation (the
+// dropped lo*lo term is below 2^-16 relative).  LDS tiles are [row][k]
+// with k contiguous so that a lane's 8 consecutive k of one row (its MFMA
+// operand) are one 16-byte read; rows are padded to 80 bytes, which keeps
+// the 16 rows read by a quarter-wave on distinct banks.
+constexpr int BK3 = 32;        // K depth of one LDS stage = one bf16 MFMA
+constexpr int LDK3 = BK3 + 8;  // bf16 elements per LDS row
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_rne(uint32_t u) {  // upper 16 bits, round to nearest even
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ void split_bf16(double x, unsigned short &hi, unsigned short &lo) {
+  const float f = float(x);
+  const uint32_t h = bf16_rne(__float_as_uint(f));
+  const float rem = f - __uint_as_float(h << 16);
+  hi = (unsigned short)h;
+  lo = (unsigned short)bf16_rne(__float_as_uint(rem));
+}
+
+template <int BR, bool KC, int NTH>
+struct TileLoader3 {
+  static constexpr int PER = BR * BK3 / NTH;
+  static_assert(PER >= 1, "tile too small for the thread count");
+  static constexpr int NK = KC ? 1 : PER;
+  double r[PER];
+  bool rok[PER];
+  int64_t roff[KC ? PER : 1];
+  bool rv[KC ? PER : 1];
+  int64_t ko[NK];
+  bool kv[NK];
+  int lr, lk;
+
+  __device__ __forceinline__ void init(int tid, int row0, int R, const int64_t *__restrict__ tabR) {
+    if constexpr (KC) {
+      lk = tid % BK3;
+      lr = tid / BK3;
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        const int row = row0 + lr + (NTH / BK3) * p;
+        rv[p] = row < R;
+        roff[p] = tabR[rv[p] ? row : 0];
+      }
+    } else {
+      lr = tid % BR;
+      lk = tid / BR;
+      const int row = row0 + lr;
+      rv[0] = row < R;
+      roff[0] = tabR[rv[0] ? row : 0];
+    }
+  }
+  __device__ __forceinline__ void fetch_k(int k0, int K, const int64_t *__restrict__ tabK) {
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+      const int k = k0 + lk + (KC ? 0 : (NTH / BR) * q);
+      kv[q] = k < K;
+      ko[q] = tabK[kv[q] ? k : 0];
+    }
+  }
+  __device__ __forceinline__ void load(const double *__restrict__ base) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const bool ok = KC ? (kv[0] && rv[p]) : (kv[p] && rv[0]);
+      const int64_t off = (KC ? roff[p] + ko[0] : roff[0] + ko[p]);
+      r[p] = base[ok ? off : 0];
+      rok[p] = ok;
+    }
+  }
+  __device__ __forceinline__ void store(unsigned short *__restrict__ sh, unsigned short *__restrict__ sl) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = KC ? (lr + (NTH / BK3) * p) * LDK3 + lk : lr * LDK3 + lk + (NTH / BR) * p;
+      unsigned short h = 0, l = 0;
+      if (rok[p]) split_bf16(r[p], h, l);
+      sh[idx] = h;
+      sl[idx] = l;
+    }
+  }
+};
+
+template <int BM, int BN, int GM, int GN, bool AKC, bool BKC>
+__global__ __launch_bounds__(64 * GM * GN) void k_gemm3(const double *__restrict__ A, const double *__restrict__ B,
+                                                        double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                        const WorkUnit *__restrict__ units, int nunits,
+                                                        const int64_t *__restrict__ tab, double alpha, double beta) {
+  constexpr int NTH = 64 * GM * GN;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  using LA = TileLoader3<BM, AKC, NTH>;
+  using LB = TileLoader3<BN, BKC, NTH>;
+  constexpr int WM = BM / GM, WN = BN / GN, FM = WM / 16, FN = WN / 16;
+  static_assert(FM >= 1 && FN >= 1, "wave tile below one MFMA fragment");
+  __shared__ __align__(16) unsigned short Ah[2][BM * LDK3], Al[2][BM * LDK3];
+  __shared__ __align__(16) unsigned short Bh[2][BN * LDK3], Bl[2][BN * LDK3];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % GM, wn = wave / GM;
+  const int G = gridDim.x, b = blockIdx.x;
+  int ustart, uend, ustep;
+  if (G % 8 == 0 && nunits >= G) {
+    const int nb = G / 8, x = b % 8, lb = b / 8;
+    const int chunk = (nunits + 7) / 8;
+    ustart = x * chunk + lb;
+    uend = min(nunits, (x + 1) * chunk);
+    ustep = nb;
+  } else {
+    ustart = b;
+    uend = nunits;
+    ustep = G;
+  }
+
+  for (int u = ustart; u < uend; u += ustep) {
+    const WorkUnit wu = units[u];
+    v4 acc[FN][FM];
+#pragma unroll
+    for (int i = 0; i < FN; ++i)
+#pragma unroll
+      for (int j = 0; j < FM; ++j) acc[i][j] = v4{0, 0, 0, 0};
+
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      if (tk.K <= 0) continue;
+      LA la;
+      LB lb;
+      la.init(tid, wu.m0, tk.M, tab + tk.tAM);
+      lb.init(tid, wu.n0, tk.N, tab + tk.tBN);
+      const double *Ab = A + tk.a_off;
+      const double *Bb = B + tk.b_off;
+      const int64_t *tAK = tab + tk.tAK;
+      const int64_t *tBK = tab + tk.tBK;
+      la.fetch_k(0, tk.K, tAK);
+      lb.fetch_k(0, tk.K, tBK);
+      la.load(Ab);
+      lb.load(Bb);
+      la.fetch_k(BK3, tk.K, tAK);
+      lb.fetch_k(BK3, tk.K, tBK);
+      la.store(Ah[0], Al[0]);
+      lb.store(Bh[0], Bl[0]);
+      __syncthreads();
+      int cur = 0;
+      for (int k0 = 0; k0 < tk.K; k0 += BK3) {
+        const bool nxt = k0 + BK3 < tk.K;
+        if (nxt) {
+          la.load(Ab);
+          lb.load(Bb);
+          la.fetch_k(k0 + 2 * BK3, tk.K, tAK);
+          lb.fetch_k(k0 + 2 * BK3, tk.K, tBK);
+        }
+        const int k8 = 8 * (lane >> 4);
+        bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int j = 0; j < FM; ++j) {
+          const int m = wm * WM + j * 16 + (lane & 15);
+          ah[j] = *reinterpret_cast<const bf16x8 *>(&Ah[cur][m * LDK3 + k8]);
+          al[j] = *reinterpret_cast<const bf16x8 *>(&Al[cur][m * LDK3 + k8]);
+        }
+#pragma unroll
+        for (int i = 0; i < FN; ++i) {
+          const int n = wn * WN + i * 16 + (lane & 15);
+          bh[i] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][n * LDK3 + k8]);
+          bl[i] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][n * LDK3 + k8]);
+        }
+        // D^T tile as in k_gemm; the two cross terms first, then hi*hi
+#pragma unroll
+        for (int i = 0; i < FN; ++i)
+#pragma unroll
+          for (int j = 0; j < FM; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[i], al[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[i], ah[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[i], ah[j], acc[i][j], 0, 0, 0);
+          }
+        if (nxt) {
+          la.store(Ah[cur ^ 1], Al[cur ^ 1]);
+          lb.store(Bh[cur ^ 1], Bl[cur ^ 1]);
+        }
+        __syncthreads();
+        cur ^= 1;
+      }
+    }
+
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int64_t *tCM = tab + t0.tCM;
+    const int64_t *tCN = tab + t0.tCN;
+    double *Cb = C + t0.c_off;
+#pragma unroll
+    for (int j = 0; j < FM; ++j) {
+      const int m = wu.m0 + wm * WM + j * 16 + (lane & 15);
+      if (m >= t0.M) continue;
+      const int64_t cm = tCM[m];
+#pragma unroll
+      for (int i = 0; i < FN; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = wu.n0 + wn * WN + i * 16 + 4 * (lane >> 4) + r;
+          if (n < t0.N) {
+            const int64_t ad = cm + tCN[n];
+            double v = alpha * double(acc[i][j][r]);
+            if (beta != 0.0) v += beta * Cb[ad];
+            Cb[ad] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
 // Skinny contractions (N <= 32 and K <= 64 for every task, e.g. the
 // MPO-tensor stages of H*psi): HBM-bound, so one thread owns one row m of
 // a 256-row unit and accumulates its N outputs in registers (FP64 FMA)
@@ -592,6 +801,29 @@ const void *gemm_fn(bool akc, bool bkc) {
              : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false>);
 }
 
+template <int BM, int BN, int GM, int GN>
+const void *gemm3_fn(bool akc, bool bkc) {
+  if (akc)
+    return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true>)
+               : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, false>);
+  return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, true>)
+             : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, false>);
+}
+
+const void *gemm3_fn_tile(int tile, bool akc, bool bkc) {
+  switch (tile) {
+    case 0: return gemm3_fn<128, 128, 2, 2>(akc, bkc);
+    case 1: return gemm3_fn<128, 64, 2, 2>(akc, bkc);
+    case 2: return gemm3_fn<64, 128, 2, 2>(akc, bkc);
+    case 3: return gemm3_fn<64, 64, 2, 2>(akc, bkc);
+    case 6: return gemm3_fn<128, 128, 2, 4>(akc, bkc);
+    case 7: return gemm3_fn<256, 64, 4, 2>(akc, bkc);
+    case 8: return gemm3_fn<64, 256, 2, 4>(akc, bkc);
+    case 9: return gemm3_fn<256, 128, 4, 2>(akc, bkc);
+    default: return gemm3_fn<32, 32, 2, 2>(akc, bkc);
+  }
+}
+
 template <typename T>
 const void *gemm_fn_tile(int tile, bool akc, bool bkc) {
   switch (tile) {
@@ -621,6 +853,7 @@ const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block) {
                : reinterpret_cast<const void *>(&k_small<false, false>);
   }
   *block = 64 * kTiles[tile].gm * kTiles[tile].gn;
+  if (prec == ITX_PREC_BF16X3) return gemm3_fn_tile(tile, akc, bkc);
   return prec == ITX_PREC_F64 ? gemm_fn_tile<double>(tile, akc, bkc) : gemm_fn_tile<float>(tile, akc, bkc);
 }
 
@@ -676,8 +909,8 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
                const int32_t *labC, int ntasks, const int64_t *offs, const int64_t *blkA,
                const int64_t *blkB, const int64_t *blkC, int prec, itx_plan **out) {
   *out = nullptr;
-  if (prec != ITX_PREC_F64 && prec != ITX_PREC_F32)
-    return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: precision form not built");
+  if (prec != ITX_PREC_F64 && prec != ITX_PREC_F32 && prec != ITX_PREC_BF16X3)
+    return set_err(ctx, ITX_ERR_ARG, "contraction: unknown precision form");
   if (ntasks < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: negative task count");
   LabelMap lm;
   int rc = map_labels(ctx, rA, labA, rB, labB, rC, labC, lm);

@@ -36,6 +36,9 @@ remove_qns = _h.remove_qns
 combiner = _h.combiner
 diag_hermitian = _h.diag_hermitian
 denmat_decomp = _h.denmat_decomp
+svd = _h.svd
+svd_spec = _h.svd_spec
+factor = _h.factor
 truncate = _h.truncate
 davidson_bond = _h.davidson_bond
 local_product = _h.local_product

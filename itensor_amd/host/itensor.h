@@ -366,6 +366,14 @@ std::tuple<Real, Real, Real, int> truncate(std::vector<Real> &P, long maxdim, lo
                                            bool absoluteCutoff, bool doRelCutoff, Args const &args);
 // diag_hermitian (hermitian.cc:40-440)
 Spectrum diag_hermitian(ITensor rho, ITensor &U, ITensor &D, Args const &args);
+// svd (decomp.cc:121-248 -> svd.cc:39-436): U (or V) names the indices of
+// AA that go to U; Args Cutoff, MaxDim, MinDim, Truncate, LeftTags ("Link,U"),
+// RightTags ("Link,V"), RespectDegenerate, DoRelCutoff, AbsoluteCutoff
+Spectrum svd(ITensor const &AA, ITensor &U, ITensor &D, ITensor &V, Args args = Args());
+std::tuple<ITensor, ITensor, ITensor> svd(ITensor const &AA, IndexSet const &Uis, Args const &args = Args());
+// factor (decomp.cc:538-576): T = A * B with sqrt(D) in both
+Spectrum factor(ITensor const &T, ITensor &A, ITensor &B, Args args = Args());
+std::tuple<ITensor, ITensor> factor(ITensor const &T, IndexSet const &Ais, Args const &args = Args());
 // denmatDecomp without noise (decomp.h:316-418)
 Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args);
 

@@ -183,6 +183,20 @@ PYBIND11_MODULE(_itensor_host, m) {
     auto spec = denmatDecomp(AA, A, B, fromleft ? Fromleft : Fromright, argsFrom(args));
     return py::make_tuple(A, B, spec.eigs, spec.truncerr);
   });
+  m.def("svd", [](ITensor const &AA, std::vector<Index> const &uinds, py::dict args) {
+    auto [U, D, V] = svd(AA, uinds, argsFrom(args));
+    return py::make_tuple(U, D, V);
+  }, py::arg("AA"), py::arg("uinds"), py::arg("args") = py::dict(), "svd(AA, {U indices}, args) -> (U, D, V)");
+  m.def("svd_spec", [](ITensor const &AA, std::vector<Index> const &uinds, py::dict args) {
+    ITensor U(uinds, nullptr), D, V;
+    auto spec = svd(AA, U, D, V, argsFrom(args));
+    return py::make_tuple(U, D, V, spec.eigs, spec.truncerr);
+  });
+  m.def("factor", [](ITensor const &T, std::vector<Index> const &ainds, py::dict args) {
+    ITensor A(ainds, nullptr), B;
+    auto spec = factor(T, A, B, argsFrom(args));
+    return py::make_tuple(A, B, spec.eigs, spec.truncerr);
+  }, py::arg("T"), py::arg("ainds"), py::arg("args") = py::dict());
   m.def("truncate", [](std::vector<double> P, long maxdim, long mindim, double cutoff, bool absc, bool rel,
                        bool respect) {
     Args a;

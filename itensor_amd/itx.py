@@ -74,6 +74,8 @@ _PROTOS = {
     "itx_chain_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "itx_chain_launch": (C.c_int, [_vp, _vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]),
     "itx_chain_destroy": (C.c_int, [_vp, _vp]),
+    "itx_gesvd": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "itx_gesvd_batched": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _vp, _i64p, _vp, _i64p, _vp, _i64p, _vp]),
 }
 
 
@@ -251,6 +253,18 @@ class Context:
         wo = np.ascontiguousarray(w_offs, dtype=np.int64)
         self.check(self.L.itx_syev_batched(self.h, len(ns), ns.ctypes.data_as(_i32p), ao.ctypes.data_as(_i64p),
                                            _vp(A), wo.ctypes.data_as(_i64p), _vp(w)), "itx_syev_batched")
+
+    def gesvd(self, m: int, n: int, A: int, U: int, S: int, V: int):
+        self.check(self.L.itx_gesvd(self.h, m, n, _vp(A), _vp(U), _vp(S), _vp(V)), "itx_gesvd")
+
+    def gesvd_batched(self, ms, ns, a_offs, A: int, u_offs, U: int, s_offs, S: int, v_offs, V: int):
+        arr = [np.ascontiguousarray(x, dtype=t) for x, t in
+               ((ms, np.int32), (ns, np.int32), (a_offs, np.int64), (u_offs, np.int64), (s_offs, np.int64),
+                (v_offs, np.int64))]
+        self.check(self.L.itx_gesvd_batched(
+            self.h, len(arr[0]), arr[0].ctypes.data_as(_i32p), arr[1].ctypes.data_as(_i32p),
+            arr[2].ctypes.data_as(_i64p), _vp(A), arr[3].ctypes.data_as(_i64p), _vp(U),
+            arr[4].ctypes.data_as(_i64p), _vp(S), arr[5].ctypes.data_as(_i64p), _vp(V)), "itx_gesvd_batched")
 
     def block_copy(self, X: int, Y: int, r: int, desc, alpha=1.0, beta=0.0):
         desc = np.ascontiguousarray(desc, dtype=np.int64).ravel()

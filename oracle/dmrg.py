@@ -434,6 +434,141 @@ def diag_hermitian(rho: ITensor, args: dict):
     return Ut, probs, {"eigs": probs.copy(), "truncerr": truncerr, "newmid": dind}
 
 
+MIN_CUT = 1e-15  # global.h:61
+MAX_DIM = 5000   # global.h:62
+
+
+def _svd_desc(M: np.ndarray):
+    """Thin SVD M = U diag(s) V^T, s descending (numpy's LAPACK gesdd)."""
+    U, s, Vt = np.linalg.svd(M, full_matrices=False)
+    return U, s, Vt.T
+
+
+def svd(AA: ITensor, uinds, args: dict):
+    """svd (decomp.cc:121-248) -> svdOrd2 -> svdImpl (svd.cc:39-423), real
+    case.  uinds: the indices of AA that go to U (ITensor U(uinds) of the
+    reference).  Returns (U, D, V, spec): U(uinds, dag(L)) ... as the
+    reference lays them out, D over (L, R), spec = dict(eigs, truncerr)."""
+    args = dict(args)
+    Linds = [I for I in AA.inds if index_position(list(uinds), I) >= 0]
+    Rinds = [I for I in AA.inds if index_position(list(uinds), I) < 0]
+    AAc, Ucomb, Vcomb, ui, vi = AA, None, None, None, None
+    if Linds:
+        Ucomb, ui = combiner(Linds)
+        AAc = AAc * Ucomb
+    if Rinds:
+        Vcomb, vi = combiner(Rinds)
+        AAc = AAc * Vcomb
+    do_truncate = args.get("Truncate", ("Cutoff" in args) or ("MaxDim" in args))
+    cutoff = args.get("Cutoff", MIN_CUT)
+    maxdim = args.get("MaxDim", MAX_DIM)
+    mindim = args.get("MinDim", 1)
+    do_rel = args.get("DoRelCutoff", True)
+    absc = args.get("AbsoluteCutoff", False)
+    respect = args.get("RespectDegenerate", False)
+    ltags = args.get("LeftTags", "Link,U")
+    rtags = args.get("RightTags", "Link,V")
+    pu = index_position(AAc.inds, ui)
+    if not AAc.is_qn:
+        M = AAc.d if pu == 0 else AAc.d.T
+        UU, DD, VV = _svd_desc(M)
+        probs = DD ** 2
+        truncerr = 0.0
+        if do_truncate:
+            probs, truncerr, _, _, _ = truncate(probs, maxdim, mindim, cutoff, absc, do_rel, respect)
+        m = probs.size
+        uL, vL = Index(m, ltags), Index(m, rtags)
+        U = dense([ui, uL], UU[:, :m])
+        V = dense([vi, vL], VV[:, :m])
+        D = dense([uL, vL], np.diag(DD[:m]))
+        spec = {"eigs": DD[:m] ** 2, "truncerr": truncerr}
+    else:
+        blocks = []
+        for b, _ in AAc.offsets:
+            M = AAc.block(b)
+            if pu != 0:
+                M = M.T
+            blocks.append((b[pu], b[1 - pu], M))
+        dvecs, umats, vmats, alleig = [], [], [], []
+        for b1, b2, M in blocks:
+            UU, d, VV = _svd_desc(M)
+            dvecs.append(d)
+            umats.append(UU)
+            vmats.append(VV)
+            alleig.extend(d.tolist())
+        probs = np.array(sorted((x * x for x in alleig), reverse=True))
+        truncerr, docut_lower, docut_upper, ndegen = 0.0, -1.0, -1.0, 1
+        m = probs.size
+        if do_truncate:
+            probs, truncerr, docut_lower, docut_upper, ndegen = truncate(probs, maxdim, mindim, cutoff, absc,
+                                                                         do_rel, respect)
+            m = probs.size
+        Liq, Riq, kept = [], [], []
+        total_m = 0
+        for (b1, b2, M), d in zip(blocks, dvecs):
+            this_m = 0
+            if do_truncate:
+                while this_m < d.size and total_m < m and d[this_m] ** 2 > docut_upper:
+                    this_m += 1
+                    total_m += 1
+                while ndegen > 0 and this_m < d.size and total_m < m and d[this_m] ** 2 > docut_lower:
+                    this_m += 1
+                    total_m += 1
+                    ndegen -= 1
+            else:
+                this_m = d.size
+                total_m += this_m
+            kept.append(this_m)
+            if this_m:
+                Liq.append((ui.qn(b1), this_m))
+                Riq.append((vi.qn(b2), this_m))
+        L = Index(Liq, ltags, dir=ui.dir)
+        R = Index(Riq, rtags, dir=vi.dir)
+        U = qdense([ui, L.dag()], QN())
+        V = qdense([vi, R.dag()], QN())
+        D = qdense([L, R], QN())
+        n = 0
+        for (b1, b2, M), d, UU, VV, k in zip(blocks, dvecs, umats, vmats, kept):
+            if not k:
+                continue
+            U.block((b1, n))[...] = UU[:, :k]
+            V.block((b2, n))[...] = VV[:, :k]
+            D.block((n, n))[...] = np.diag(d[:k])
+            n += 1
+        spec = {"eigs": probs.copy(), "truncerr": truncerr}
+    if Ucomb is not None:
+        U = Ucomb.dag() * U
+    if Vcomb is not None:
+        V = V * Vcomb.dag()
+    return U, D, V, spec
+
+
+def factor(T: ITensor, ainds, args: dict):
+    """factor (decomp.cc:538-576): svd with LeftTags = Tags (default
+    "Link"), then sqrt(D) absorbed into both factors, which end up sharing
+    the left link index.  Returns (A, B, spec)."""
+    args = dict(args)
+    args["LeftTags"] = args.get("Tags", "Link")
+    A, D, B, spec = svd(T, ainds, args)
+    L, R = D.inds
+    if D.is_qn:
+        Dsq = D.copy()
+        Dsq.store = np.sqrt(np.abs(Dsq.store))
+    else:
+        Dsq = dense(D.inds, np.sqrt(np.abs(D.d)))
+    # A(.., dag(L)) * Z(L, dag(L)'), noprime; B = Dsq(L, R) * B(.., dag(R))
+    if D.is_qn:
+        Z = qdense([L, L.dag().prime()], QN())
+        for b, _ in Z.offsets:
+            if b[0] == b[1]:
+                Z.block(b)[...] = Dsq.block(b)
+    else:
+        Z = dense([L, L.prime()], np.sqrt(np.abs(D.d)))
+    A = (A * Z).noprime()
+    B = Dsq * B
+    return A, B, spec
+
+
 def common_index(A: ITensor, B: ITensor):
     for I in A.inds:
         if index_position(B.inds, I) >= 0:

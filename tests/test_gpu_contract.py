@@ -12,6 +12,8 @@ pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12   # FP64 MFMA vs FP64 BLAS: relative to max |C|
 TOL32 = 2e-5    # FP32 MFMA form
+TOLBF = 1e-4    # BF16x3 form (operands carried to 16-17 bits, FP32 accumulate)
+TOLS = {"f64": TOL64, "f32": TOL32, "bf16x3": TOLBF}
 
 
 def _run(ctx, dev, A, labA, B, labB, labC, prec="f64", alpha=1.0, beta=0.0, C0=None):
@@ -44,14 +46,16 @@ CASES = [
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
 def test_dense_cases(ctx, dev, case, prec):
     dA, labA, dB, labB, labC = CASES[case]
     rng = np.random.default_rng(100 + case)
     A = rng.standard_normal(dA)
     B = rng.standard_normal(dB)
     got, ref = _run(ctx, dev, A, labA, B, labB, labC, prec)
-    assert relerr(got, ref) < (TOL64 if prec == "f64" else TOL32), (case, prec)
+    err = relerr(got, ref)
+    print(f"[dense case] {case} {prec} err={err:.2e}", flush=True)
+    assert err < TOLS[prec], (case, prec)
 
 
 def test_random_permutations(ctx, dev):
@@ -299,3 +303,30 @@ def test_chain_with_split_plan(ctx, dev):
         chain.close()
         p1.close()
         p2.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("chi", [64, 200])
+def test_hpsi_chain_reduced_precision(ctx, dev, chi, prec):
+    """The four H*psi stages in the FP32 and BF16x3 forms: error against the
+    FP64 CPU chain within the form's tolerance (relative to max |H psi|)."""
+    rng = np.random.default_rng(chi + 7)
+    psi = rng.standard_normal((chi, 2, 2, chi))
+    L = rng.standard_normal((chi, 5, chi))
+    R = rng.standard_normal((chi, 5, chi))
+    W = heisenberg_w()
+    ref = hpsi_dense(psi, L, W, W, R)
+    x = dev.put(psi)
+    dims = psi.shape
+    for (la, lb, lc), op in zip(HPSI_STAGES, (L, W, W, R)):
+        outd = [d for d, l in zip(dims, la) if l > 0] + [d for d, l in zip(op.shape, lb) if l > 0]
+        order = [l for l in la if l > 0] + [l for l in lb if l > 0]
+        outd = [outd[order.index(l)] for l in lc]
+        y = dev.empty(int(np.prod(outd)))
+        dop = dev.put(op)
+        ctx.contract_dense(x.data_ptr(), dims, la, dop.data_ptr(), op.shape, lb, y.data_ptr(), outd, lc,
+                           1.0, 0.0, prec)
+        x, dims = y, tuple(outd)
+    err = relerr(dev.get(x, dims), ref)
+    print(f"[hpsi chain] chi={chi} {prec} err={err:.2e}", flush=True)
+    assert err < TOLS[prec]

@@ -257,3 +257,83 @@ def test_tfim_n32_closed_form_gpu(H):
     E, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {})
     exact = (1.0 - 1.0 / math.sin(math.pi / (2 * (2 * N + 1)))) / 4
     assert abs((E - exact) / exact) < 1e-5
+
+
+# ---- svd / factor (decomp.cc:121-248, 538-576; svd.cc:39-436) ---------
+def _recon(H, U, D, V):
+    return H.remove_qns(U * D * V).store() if U.is_qn else (U * D * V).store()
+
+
+def test_svd_dense_cases_match_oracle(H):
+    """decomp_test.cc:140-181 (cases 1-3): U*D*V reconstructs T; U holds the
+    requested indices, V the rest; singular values as the oracle's."""
+    rng = np.random.default_rng(21)
+    i, j, k = Index(3, "i"), Index(4, "j"), Index(5, "k")
+    T = dense([i, j, k], rng.standard_normal((3, 4, 5)))
+    T3 = dense([i, k, i.prime()], rng.standard_normal((3, 5, 3)))
+    for TT, uinds in ((T, [i, j]), (T, [i, k]), (T3, [i, i.prime()])):
+        Uo, Do, Vo, so = OD.svd(TT, uinds, {})
+        U, D, V, eigs, terr = H.svd_spec(cten(TT), [cidx(x) for x in uinds], {})
+        ref = cten(TT)
+        assert H.norm(U * D * V - ref) < 1e-12 * H.norm(ref)
+        assert np.allclose(eigs, so["eigs"], rtol=0, atol=1e-12 * so["eigs"][0])
+        assert {x.id for x in U.inds} >= {x.id for x in uinds}
+        A, B, _, _ = H.factor(cten(TT), [cidx(x) for x in uinds], {})
+        assert H.norm(A * B - ref) < 1e-12 * H.norm(ref)
+
+
+def test_svd_qn_regressions_and_structure(H):
+    """decomp_test.cc:184-225 (QN regression tests) plus a random flux-0
+    block tensor: reconstruction, and L/R sector lists, arrows and the U,
+    V, D block lists identical to the oracle's."""
+    from oracle.itensor import QN as OQN
+
+    rng = np.random.default_rng(8)
+    u = Index([(OQN(2), 1), (OQN(0), 1), (OQN(-2), 1)], "u")
+    v = Index([(OQN(2), 1), (OQN(0), 1), (OQN(-2), 1)], "v")
+    S = qdense([u, v], OQN())
+    S.store = rng.standard_normal(S.store.size)
+    a = Index([(OQN(1), 3), (OQN(-1), 4), (OQN(3), 2)], "a")
+    b = Index([(OQN(0), 2), (OQN(2), 3), (OQN(-2), 3)], "b")
+    c = Index([(OQN(1), 2), (OQN(-1), 2)], "c")
+    T = qdense([a, b.dag(), c], OQN(1))
+    T.store = rng.standard_normal(T.store.size)
+    for TT, uinds, args in ((S, [u], {}), (T, [a, c], {}), (T, [a, c], {"MaxDim": 5, "Cutoff": 1e-3}),
+                            (T, [b.dag()], {"Cutoff": 1e-2})):
+        Uo, Do, Vo, so = OD.svd(TT, uinds, args)
+        U, D, V, eigs, terr = H.svd_spec(cten(TT), [cidx(x) for x in uinds], args)
+        for got, ref in ((U, Uo), (D, Do), (V, Vo)):
+            assert [(x.dim, x.dir, x.plev) for x in got.inds] == [(x.dim, x.dir, x.plev) for x in ref.inds]
+            assert [(tuple(x), o) for x, o in got.offsets] == [(x, o) for x, o in ref.offsets]
+        assert D.inds[0].qns == cidx(Do.inds[0]).qns and D.inds[1].qns == cidx(Do.inds[1]).qns
+        assert np.allclose(eigs, so["eigs"], rtol=0, atol=1e-12 * so["eigs"][0])
+        assert abs(terr - so["truncerr"]) < 1e-12
+        if "MaxDim" not in args and "Cutoff" not in args:
+            ref = cten(TT)
+            assert H.norm(U * D * V - ref) < 1e-12 * H.norm(ref)
+        A, B, _, _ = H.factor(cten(TT), [cidx(x) for x in uinds], args)
+        Ao, Bo, _ = OD.factor(TT, uinds, args)
+        AB = (A * B).store()
+        ABo = cten(Ao * Bo).store()
+        assert np.max(np.abs(AB - ABo)) < 1e-12 * max(1.0, np.max(np.abs(ABo)))
+
+
+@pytest.mark.parametrize("args,dim", [
+    ({"MaxDim": 9, "Cutoff": 0.0}, 9), ({"MaxDim": 9, "Cutoff": 0.0, "RespectDegenerate": True}, 4),
+    ({"MinDim": 9, "Cutoff": 1.0}, 9), ({"MinDim": 9, "Cutoff": 1.0, "RespectDegenerate": True}, 16),
+    ({"Cutoff": 0.25}, 10), ({"Cutoff": 0.25, "RespectDegenerate": True}, 16),
+])
+def test_svd_degenerate_truncation_known_answers(H, args, dim):
+    """decomp_test.cc:881-1030: kept dimension of svd with and without QNs."""
+    from oracle.itensor import QN as OQN, remove_qns
+
+    vals = [2.0, 1.0, 1.0, 1.0, 0.5]
+    iq = Index([(OQN(b), 5) for b in range(4)], "i")
+    A = qdense([iq, iq.prime().dag()], OQN())
+    for bb, _ in A.offsets:
+        if bb[0] == bb[1]:
+            A.block(bb)[...] = np.diag(vals)
+    A.store = A.store / np.linalg.norm(A.store)
+    for TT, ind in ((A, iq), (remove_qns(A), remove_qns(A).inds[0])):
+        U, D, V = H.svd(cten(TT), [cidx(ind)], args)
+        assert D.inds[0].dim == dim and D.inds[1].dim == dim

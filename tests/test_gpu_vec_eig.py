@@ -118,3 +118,61 @@ def test_syev_batched_mixed_classes(ctx, dev):
         assert np.max(np.abs(wn - wref)) / scale < 1e-12, n
         assert np.max(np.abs(U.T @ U - np.eye(n))) < 1e-12, n
         assert np.max(np.abs(U @ np.diag(wn) @ U.T - X)) / scale < 1e-12, n
+
+
+@pytest.mark.parametrize("m,n,kind", [(1, 1, "rand"), (1, 7, "rand"), (7, 1, "rand"), (5, 9, "rand"),
+                                      (9, 5, "rand"), (40, 40, "rank"), (64, 130, "rand"), (130, 64, "rand"),
+                                      (100, 100, "decay"), (33, 200, "decay")])
+def test_gesvd(ctx, dev, m, n, kind):
+    """One-sided Jacobi SVD vs numpy: singular values (descending) to 1e-12
+    of the largest, orthonormal singular vectors, reconstruction to 1e-13
+    relative -- the accuracy stress test of matrix_test.cc:1808-1841 is
+    the 'decay' case (spectrum 0.8^j)."""
+    rng = np.random.default_rng(m * 1000 + n)
+    M = rng.standard_normal((m, n))
+    k = min(m, n)
+    if kind == "rank":
+        M = rng.standard_normal((m, 7)) @ rng.standard_normal((7, n))
+    elif kind == "decay":
+        Uq, _ = np.linalg.qr(rng.standard_normal((m, k)))
+        Vq, _ = np.linalg.qr(rng.standard_normal((n, k)))
+        M = (Uq * 0.8 ** np.arange(k)) @ Vq.T
+    dA = dev.put(M)
+    dU, dS, dV = dev.empty(m * k), dev.empty(k), dev.empty(n * k)
+    ctx.gesvd(m, n, dA.data_ptr(), dU.data_ptr(), dS.data_ptr(), dV.data_ptr())
+    U, S, V = dev.get(dU, (m, k)), dev.get(dS, (k,)), dev.get(dV, (n, k))
+    ref = np.linalg.svd(M, compute_uv=False)
+    assert np.all(np.diff(S) <= 0)
+    assert np.max(np.abs(S - ref)) < 1e-12 * max(1.0, ref[0])
+    assert np.linalg.norm(U @ np.diag(S) @ V.T - M) / np.linalg.norm(M) < 1e-13
+    nz = S > 1e-10 * S[0]
+    for X in (U[:, nz], V[:, nz]):
+        assert np.max(np.abs(X.T @ X - np.eye(X.shape[1]))) < 1e-12
+    assert np.array_equal(dev.get(dA, (m, n)), M)  # input untouched
+
+
+def test_gesvd_batched(ctx, dev):
+    """Several blocks of different shapes in one launch, stored back to back."""
+    rng = np.random.default_rng(5)
+    shapes = [(3, 5), (8, 2), (1, 1), (20, 20), (0, 4), (6, 6)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    ks = [min(s) for s in shapes]
+    a_offs = np.cumsum([0] + [a * b for a, b in shapes])[:-1]
+    u_offs = np.cumsum([0] + [a * k for (a, _), k in zip(shapes, ks)])[:-1]
+    v_offs = np.cumsum([0] + [b * k for (_, b), k in zip(shapes, ks)])[:-1]
+    s_offs = np.cumsum([0] + ks)[:-1]
+    A = np.concatenate([x.ravel(order="F") for x in mats] + [np.zeros(1)])
+    dA = dev.put(A)
+    nU = sum(a * k for (a, _), k in zip(shapes, ks)) + 1
+    nV = sum(b * k for (_, b), k in zip(shapes, ks)) + 1
+    dU, dS, dV = dev.empty(nU), dev.empty(sum(ks) + 1), dev.empty(nV)
+    ctx.gesvd_batched([s[0] for s in shapes], [s[1] for s in shapes], a_offs, dA.data_ptr(), u_offs,
+                      dU.data_ptr(), s_offs, dS.data_ptr(), v_offs, dV.data_ptr())
+    Uh, Sh, Vh = dev.get(dU, (nU,)), dev.get(dS, (sum(ks) + 1,)), dev.get(dV, (nV,))
+    for M, (a, b), k, uo, so, vo in zip(mats, shapes, ks, u_offs, s_offs, v_offs):
+        if k == 0:
+            continue
+        U = Uh[uo: uo + a * k].reshape((a, k), order="F")
+        V = Vh[vo: vo + b * k].reshape((b, k), order="F")
+        S = Sh[so: so + k]
+        assert np.linalg.norm(U @ np.diag(S) @ V.T - M) < 1e-12 * max(1.0, np.linalg.norm(M))

@@ -11,6 +11,7 @@ import pytest
 from oracle.dmrg import (LocalMPO, SplitMix64, davidson, denmat_decomp, diag_hermitian, dmrg, heisenberg_mpo,
                          product_mps, random_mps, spin_half_sites, tfim_mpo, truncate)
 from oracle.itensor import QN, Index, contract, dense, norm, qdense
+from oracle import dmrg as OD
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "heisenberg_ed.json")))
 
@@ -164,3 +165,81 @@ def test_tfim_critical_n32_closed_form():
     E, _ = dmrg(H, psi0, sw)
     exact = (1.0 - 1.0 / math.sin(math.pi / (2 * (2 * N + 1)))) / 4
     assert abs((E - exact) / exact) < 1e-5  # CHECK_CLOSE: |x| < 1e-5
+
+
+# ---- svd / factor (decomp.cc:121-248, 538-576; svd.cc:39-423) ----------
+def _degenerate_case():
+    """decomp_test.cc:881-901: blocks QN(0..3) of dim 5 holding
+    diag(2,1,1,1,0.5), normalised."""
+    vals = [2.0, 1.0, 1.0, 1.0, 0.5]
+    iq = Index([(QN(b), 5) for b in range(4)], "i")
+    A = qdense([iq, iq.prime().dag()], QN())
+    for b, _ in A.offsets:
+        if b[0] == b[1]:
+            A.block(b)[...] = np.diag(vals)
+    A.store = A.store / np.linalg.norm(A.store)
+    return iq, A
+
+
+@pytest.mark.parametrize("args,qn_dim,dense_dim", [
+    ({"MaxDim": 9, "Cutoff": 0.0}, 9, 9),                               # decomp_test.cc:905-940
+    ({"MaxDim": 9, "Cutoff": 0.0, "RespectDegenerate": True}, 4, 4),
+    ({"MinDim": 9, "Cutoff": 1.0}, 9, 9),                               # :945-985
+    ({"MinDim": 9, "Cutoff": 1.0, "RespectDegenerate": True}, 16, 16),
+    ({"Cutoff": 0.25}, 10, 10),                                          # :990-1025
+    ({"Cutoff": 0.25, "RespectDegenerate": True}, 16, 16),
+])
+def test_oracle_svd_degenerate_truncation_known_answers(args, qn_dim, dense_dim):
+    from oracle.itensor import remove_qns
+
+    iq, A = _degenerate_case()
+    U, D, V, spec = OD.svd(A, [iq], args)
+    assert D.inds[0].dim == D.inds[1].dim == qn_dim
+    Ad = remove_qns(A)
+    U, D, V, spec = OD.svd(Ad, [Ad.inds[0]], args)
+    assert D.inds[0].dim == D.inds[1].dim == dense_dim
+
+
+def test_oracle_svd_special_cases():
+    """decomp_test.cc:756-800: all-zero blocks and a dimension-one index
+    keep one singular value with Cutoff 1."""
+    iq = Index([(QN(b), 2) for b in range(4)], "i")
+    A = qdense([iq, iq.prime().dag()], QN())
+    U, D, V, spec = OD.svd(A, [iq], {"Cutoff": 1.0})
+    assert D.inds[0].dim == 1 and D.inds[1].dim == 1
+    i1 = Index([(QN(), 1)], "i")
+    A = qdense([i1, i1.prime().dag()], QN())
+    A.store = np.array([0.7])
+    U, D, V, spec = OD.svd(A, [i1], {"Cutoff": 1.0})
+    assert D.inds[0].dim == 1
+
+
+def test_oracle_svd_reconstruction_and_factor():
+    """decomp_test.cc:140-225: U*D*V and A*B reconstruct T (dense cases 1-3
+    and the QN regression tests); U, V have orthonormal columns."""
+    from oracle.itensor import norm
+
+    rng = np.random.default_rng(4)
+    i, j, k = Index(3, "i"), Index(4, "j"), Index(5, "k")
+    T = dense([i, j, k], rng.standard_normal((3, 4, 5)))
+    for uinds in ([i, j], [i, k]):
+        U, D, V, spec = OD.svd(T, uinds, {})
+        assert norm(U * D * V - T) < 1e-12
+        A, B, _ = OD.factor(T, uinds, {})
+        assert norm(A * B - T) < 1e-12
+    T3 = dense([i, k, i.prime()], rng.standard_normal((3, 5, 3)))
+    U, D, V, _ = OD.svd(T3, [i, i.prime()], {})
+    assert norm(U * D * V - T3) < 1e-12
+    u = Index([(QN(2), 1), (QN(0), 1), (QN(-2), 1)], "u")
+    v = Index([(QN(2), 1), (QN(0), 1), (QN(-2), 1)], "v")
+    S = qdense([u, v], QN())
+    S.store = rng.standard_normal(S.store.size)
+    U, D, V, _ = OD.svd(S, [u], {})
+    assert norm(U * D * V - S) < 1e-12
+    # singular values equal numpy's on the dense expansion
+    from oracle.itensor import remove_qns
+
+    M = rng.standard_normal((6, 9))
+    a, b = Index(6, "a"), Index(9, "b")
+    U, D, V, spec = OD.svd(dense([a, b], M), [a], {})
+    assert np.allclose(np.sqrt(spec["eigs"]), np.linalg.svd(M, compute_uv=False), atol=1e-13)
