@@ -296,8 +296,9 @@ def test_svd_qn_regressions_and_structure(H):
     a = Index([(OQN(1), 3), (OQN(-1), 4), (OQN(3), 2)], "a")
     b = Index([(OQN(0), 2), (OQN(2), 3), (OQN(-2), 3)], "b")
     c = Index([(OQN(1), 2), (OQN(-1), 2)], "c")
-    T = qdense([a, b.dag(), c], OQN(1))
+    T = qdense([a, b.dag(), c], OQN(0))
     T.store = rng.standard_normal(T.store.size)
+    assert T.nnzblocks() > 4
     for TT, uinds, args in ((S, [u], {}), (T, [a, c], {}), (T, [a, c], {"MaxDim": 5, "Cutoff": 1e-3}),
                             (T, [b.dag()], {"Cutoff": 1e-2})):
         Uo, Do, Vo, so = OD.svd(TT, uinds, args)
