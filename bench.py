@@ -13,8 +13,16 @@ runs the reference's four separate stages instead).
 value = algorithmic FP64-equivalent flops (80 chi^3 + 800 chi^2 per H*psi)
         / wall time, whole job (one GPU).
 
+Further legs on the same line: "qn_hpsi" -- the block-sparse H*psi at
+chi = 1024 over 31 Sz sectors in every execution mode (grouped chain as a
+HIP graph / launched directly / eager, one launch per block pair, FP32 and
+BF16x3 forms); "dmrg" -- QN Heisenberg DMRG N=100 with the maxdim ramp to
+2048 (cutoff 1e-10) in the FP64, FP32 and BF16x3 H*psi forms: seconds per
+sweep, energies and their differences.
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 1024]
-                       [--no-cpu] [--prec f64|f32] [--chain fused|4stage]
+                       [--no-cpu] [--no-qn] [--dmrg N:maxdims|''] [--dmrg-forms f64,f32,bf16x3]
+                       [--prec f64|f32|bf16x3] [--chain fused|4stage]
 """
 from __future__ import annotations
 
@@ -244,10 +252,11 @@ def heisenberg_qn_chain(H, N: int):
 DMRG_MAXDIM = [10, 20, 50, 100, 200, 400, 800, 1200, 1600, 2048]
 
 
-def dmrg_leg(N: int, maxdims, prec: int, mode: int):
+def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0):
     """QN Heisenberg DMRG (N sites, Neel start, one sweep per maxdim entry,
-    cutoff 1e-10, Davidson niter 2) in one H*psi form; wall time per sweep
-    (everything included) and the phase split of each sweep."""
+    cutoff 1e-10, Davidson niter 2) in one H*psi form: wall time of each
+    sweep (everything included) and how it splits over the phases of the
+    bond steps (environment update, davidson, svdBond)."""
     import torch
 
     from itensor_amd import host as H
@@ -256,16 +265,17 @@ def dmrg_leg(N: int, maxdims, prec: int, mode: int):
     sweeps = [{"MaxDim": int(m), "Cutoff": 1e-10, "Niter": 2} for m in maxdims]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "HpsiMode": mode, "SyncTiming": True})
+    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "HpsiMode": mode, "SyncTiming": True,
+                                               "FinalFP64Sweeps": final_fp64})
     wall = time.perf_counter() - t0
-    per = []
-    for k in range(len(maxdims)):
-        rs = [r for r in recs if r["sweep"] == k + 1]
-        per.append({"maxdim": int(maxdims[k]), "sec": round(secs[k], 4), "dim": max(r["dim"] for r in rs),
-                    "env": round(sum(r["t_env"] for r in rs), 4), "davidson": round(sum(r["t_davidson"] for r in rs), 4),
-                    "svd": round(sum(r["t_svd"] for r in rs), 4), "products": sum(r["nprod"] for r in rs),
-                    "truncerr_max": max(r["truncerr"] for r in rs), "energy": rs[-1]["energy"]})
-    return {"energy": E, "wall_s": round(wall, 3), "sweeps": per}
+    last = [r for r in recs if r["sweep"] == len(maxdims)]
+    return {"energy": E, "wall_s": round(wall, 3), "sec_per_sweep": [round(x, 4) for x in secs],
+            "max_dim": [max(r["dim"] for r in recs if r["sweep"] == k + 1) for k in range(len(maxdims))],
+            "truncerr_max_last_sweep": max(r["truncerr"] for r in last),
+            "last_sweep_split_s": {"env": round(sum(r["t_env"] for r in last), 4),
+                                   "davidson": round(sum(r["t_davidson"] for r in last), 4),
+                                   "svd": round(sum(r["t_svd"] for r in last), 4)},
+            "hpsi_products_last_sweep": sum(r["nprod"] for r in last)}
 
 
 def copy_calibration(gib: float):
@@ -295,8 +305,10 @@ def main():
     ap.add_argument("--chain", default="fused", choices=["fused", "4stage"],
                     help="LocalOp::product as run by the host layer (MPO pair fused) or the 4-stage chain")
     ap.add_argument("--no-qn", action="store_true", help="skip the block-sparse (31 Sz sectors) H*psi leg")
-    ap.add_argument("--dmrg", default="", help="run the QN DMRG leg: N:maxdim1,maxdim2,... (e.g. 100:full)")
-    ap.add_argument("--dmrg-forms", default="f64,f32", help="H*psi forms of the DMRG leg")
+    ap.add_argument("--dmrg", default="100:full",
+                    help="QN DMRG leg: N:maxdim1,maxdim2,... ('full' = the 10-sweep ramp to 2048); '' skips it")
+    ap.add_argument("--dmrg-forms", default="f64,f32,bf16x3,bf16x3+f64last",
+                    help="H*psi forms of the DMRG leg ('+f64last': the last sweep in FP64)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
@@ -424,15 +436,19 @@ def main():
     if args.dmrg:
         n_s, md_s = args.dmrg.split(":")
         mds = DMRG_MAXDIM if md_s == "full" else [int(x) for x in md_s.split(",")]
-        line["dmrg"] = {"workload": "QN Heisenberg N=%s open chain, Neel start, maxdim %s, cutoff 1e-10, niter 2"
-                                    % (n_s, mds)}
+        dm = {"workload": "QN Heisenberg N=%s open chain (total Sz 0, Neel start), maxdim %s, cutoff 1e-10, "
+                          "niter 2; H*psi in the named form, everything else FP64" % (n_s, mds)}
         for form in args.dmrg_forms.split(","):
-            line["dmrg"][form] = dmrg_leg(int(n_s), mds, {"f64": 0, "f32": 1, "bf16x3": 2}[form], 2)
-            print("dmrg leg %s done in %.1f s" % (form, line["dmrg"][form]["wall_s"]), file=sys.stderr, flush=True)
-        if "f64" in line["dmrg"] and "f32" in line["dmrg"]:
-            a, b = line["dmrg"]["f64"], line["dmrg"]["f32"]
-            line["dmrg"]["dE_f32_vs_f64"] = b["energy"] - a["energy"]
-            line["dmrg"]["speedup_f32_vs_f64"] = round(a["wall_s"] / b["wall_s"], 3)
+            base, _, last = form.partition("+")
+            dm[form] = dmrg_leg(int(n_s), mds, {"f64": 0, "f32": 1, "bf16x3": 2}[base], 2,
+                                1 if last == "f64last" else 0)
+            print("dmrg leg %s done in %.1f s" % (form, dm[form]["wall_s"]), file=sys.stderr, flush=True)
+        if "f64" in dm:
+            for form in args.dmrg_forms.split(","):
+                if form != "f64":
+                    dm["dE_%s_vs_f64" % form] = dm[form]["energy"] - dm["f64"]["energy"]
+                    dm["sweep_speedup_%s_vs_f64" % form] = round(dm["f64"]["wall_s"] / dm[form]["wall_s"], 3)
+        line["dmrg"] = dm
 
     if not args.no_cpu:
         ref, t_cpu, threads, reps = cpu_baseline(psi, L, W, R)

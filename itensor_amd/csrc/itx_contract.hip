@@ -31,7 +31,7 @@
 namespace itx {
 namespace {
 
-constexpr int BK = 16;   // K depth of one LDS stage
+constexpr int BK = 16;   // K depth of one LDS stage (default)
 
 template <typename T>
 struct MF;
@@ -70,7 +70,7 @@ struct MF<float> {
 // (out-of-range elements read element 0 of the block and are zeroed), which
 // keeps the load stream free of exec-mask branches.
 // Offsets stay 64-bit: 32-bit offsets measured 1.6x slower on gfx950.
-template <typename T, int BR, bool KC, int NTH>
+template <typename T, int BR, bool KC, int NTH, int BK = 16>
 struct TileLoader {
   static constexpr int PER = BR * BK / NTH;  // elements per thread
   static_assert(PER >= 1, "tile too small for the thread count");
@@ -140,15 +140,15 @@ struct TileLoader {
 
 // One workgroup = GM x GN waves computing a BM x BN tile of C; each wave
 // owns a (BM/GM) x (BN/GN) sub-tile of 16x16 MFMA fragments.
-template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC>
+template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC, int BK>
 __global__ __launch_bounds__(64 * GM * GN) void k_gemm(const double *__restrict__ A, const double *__restrict__ B,
                                                        double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                        const WorkUnit *__restrict__ units, int nunits,
                                                        const int64_t *__restrict__ tab, double alpha, double beta) {
   constexpr int NTH = 64 * GM * GN;
   using v4 = typename MF<T>::v4;
-  using LA = TileLoader<T, BM, AKC, NTH>;
-  using LB = TileLoader<T, BN, BKC, NTH>;
+  using LA = TileLoader<T, BM, AKC, NTH, BK>;
+  using LB = TileLoader<T, BN, BKC, NTH, BK>;
   constexpr int WM = BM / GM, WN = BN / GN, FM = WM / 16, FN = WN / 16;
   static_assert(FM >= 1 && FN >= 1, "wave tile below one MFMA fragment");
   __shared__ T As[2][BK * LA::LD];
@@ -784,21 +784,24 @@ constexpr TileCfg kTiles[] = {
     {64, 256, 2, 4, 0.95, 1.00},   // 8: 8 waves
     {256, 128, 4, 2, 0.60, 0.50},  // 9: 8 waves, wave tile 64x64
     {SM_M, SM_N, 0, 0, 1.0, 1.0},  // 10: small blocks (FP64 VALU)
+    {128, 128, 2, 4, 0.80, 0.85},  // 11: as 6 with 32-deep k stages
+    {128, 64, 2, 2, 0.90, 0.95},   // 12: as 1 with 32-deep k stages
+    {64, 128, 2, 2, 0.90, 0.95},   // 13: as 2 with 32-deep k stages
 };
-constexpr int kNumTiles = 11;
+constexpr int kNumTiles = 14;
 constexpr int kSkinny = 5;
 constexpr int kSmall = 10;
 constexpr int kSkinnyOcc = 4;
 
 // Kernel instantiation of a tile shape, operand layout and form.  All
 // kernels of the engine share one argument list (see KernelArgs).
-template <typename T, int BM, int BN, int GM, int GN>
+template <typename T, int BM, int BN, int GM, int GN, int BKS = BK>
 const void *gemm_fn(bool akc, bool bkc) {
   if (akc)
-    return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true>)
-               : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false>);
-  return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true>)
-             : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false>);
+    return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true, BKS>)
+               : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false, BKS>);
+  return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true, BKS>)
+             : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false, BKS>);
 }
 
 template <int BM, int BN, int GM, int GN>
@@ -820,6 +823,9 @@ const void *gemm3_fn_tile(int tile, bool akc, bool bkc) {
     case 7: return gemm3_fn<256, 64, 4, 2>(akc, bkc);
     case 8: return gemm3_fn<64, 256, 2, 4>(akc, bkc);
     case 9: return gemm3_fn<256, 128, 4, 2>(akc, bkc);
+    case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc);
+    case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc);
+    case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc);
     default: return gemm3_fn<32, 32, 2, 2>(akc, bkc);
   }
 }
@@ -835,6 +841,9 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc) {
     case 7: return gemm_fn<T, 256, 64, 4, 2>(akc, bkc);
     case 8: return gemm_fn<T, 64, 256, 2, 4>(akc, bkc);
     case 9: return gemm_fn<T, 256, 128, 4, 2>(akc, bkc);
+    case 11: return gemm_fn<T, 128, 128, 2, 4, 32>(akc, bkc);
+    case 12: return gemm_fn<T, 128, 64, 2, 2, 32>(akc, bkc);
+    case 13: return gemm_fn<T, 64, 128, 2, 2, 32>(akc, bkc);
     default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc);
   }
 }
@@ -1091,12 +1100,20 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     }
     if (!split) {
       vt = st;
+      // Tiles of a C block in super-tiles of SUPM x SUPN: the 32 workgroups
+      // an XCD runs side by side then share 4 row blocks of A and 8 column
+      // blocks of B in that XCD's L2 (m-major order shares one row block
+      // with 32 column blocks, re-reading B once per row block).
+      const int SUPM = 4, SUPN = 8;
       for (auto &g : groups) {
-        for (int m0 = 0; m0 < g.M; m0 += tbm)
-          for (int n0 = 0; n0 < g.N; n0 += tbn) {
-            units.push_back({g.tb, g.te, m0, n0, -1, 0});
-            ucost.push_back(g.ksum);
-          }
+        const int tm = (g.M + tbm - 1) / tbm, tn = (g.N + tbn - 1) / tbn;
+        for (int sm = 0; sm < tm; sm += SUPM)
+          for (int sn = 0; sn < tn; sn += SUPN)
+            for (int im = sm; im < std::min(tm, sm + SUPM); ++im)
+              for (int in = sn; in < std::min(tn, sn + SUPN); ++in) {
+                units.push_back({g.tb, g.te, im * tbm, in * tbn, -1, 0});
+                ucost.push_back(g.ksum);
+              }
       }
     }
     // heaviest units first (the persistent loop then balances like LPT);

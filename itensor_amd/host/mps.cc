@@ -300,6 +300,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   PH.setFusion(args.getBool("FuseMPO", true));
   PH.setMode(HpsiMode(args.getInt("HpsiMode", long(HpsiMode::Graph))));
   const bool syncTiming = args.getBool("SyncTiming", false);
+  const long finalFP64 = args.getInt("FinalFP64Sweeps", 0);
   Real energy = NAN;
   for (size_t sw = 0; sw < sweeps.size(); ++sw) {
     auto const &S = sweeps[sw];
@@ -311,6 +312,9 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
     args.add("Noise", S.noise);
     args.add("MaxIter", S.niter);
     if (S.noise != 0) throw ITError("dmrg: noise term (deltaRho) is not part of this build");
+    // precision schedule: the last FinalFP64Sweeps sweeps run the H*psi
+    // contractions in FP64 whatever HpsiPrecision says
+    if (long(sweeps.size() - sw) <= finalFP64) PH.setPrecision(ITX_PREC_F64);
     for (int b = 1, ha = 1; ha <= 2; sweepnext(b, ha, N)) {
       using clk = std::chrono::steady_clock;
       auto secs = [](clk::time_point a, clk::time_point b2) { return std::chrono::duration<double>(b2 - a).count(); };
