@@ -278,6 +278,22 @@ def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0):
             "hpsi_products_last_sweep": sum(r["nprod"] for r in last)}
 
 
+def measured_traffic(chi: int, prec: str, stage: str):
+    """HBM bytes per launch of a stage as rocprofv3 FETCH_SIZE + WRITE_SIZE
+    measured them (copy-calibrated) -- read from the committed profile
+    summary of the chi = 1024 FP64 chain; None for other workloads."""
+    if chi != 1024 or prec != "f64":
+        return None
+    path = os.path.join(ROOT, "profiles", "pmc_dense_hpsi_chi1024.md")
+    try:
+        import re
+
+        m = re.search(r"traffic_bytes_per_launch %s = (\d+)" % stage, open(path).read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def copy_calibration(gib: float):
     """A plain device-to-device copy of known size (reads and writes
     `gib` GiB per launch), the reference point for HBM byte counters."""
@@ -417,7 +433,10 @@ def main():
         "config": {"workload": "dense two-site Hpsi, chi=%d, d=2, k=5 (Heisenberg MPO)" % chi,
                    "chi": chi, "flops_per_step": flops, "chain": args.chain, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK[args.prec],
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.prec], 4),
+                     "traffic": measured_traffic(chi, args.prec, ["S1", "W12", "S4"][dom] if nst == 3 else "none"),
+                     "traffic_source": "profiles/pmc_dense_hpsi_chi1024.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                                       "dispatch, scaled by a 1 GiB copy: x2.0 / x1.0)",
                      "kernel": "k_gemm (stage %d of %d of H*psi)" % (dom + 1, nst),
                      "flops_per_launch": dom_flops, "avg_launch_ms": round(st[dom], 4)},
         "stages_ms": {args.prec: [round(x, 4) for x in st]},

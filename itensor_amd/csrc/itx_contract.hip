@@ -1248,7 +1248,9 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     ctx->tune[key] = best;
   } else {
     double best_t = 1e300;
-    for (int ti = 0; ti < kNumTiles; ++ti) {
+    // the 32-deep variants (11-13) are left to measurement (autotuner,
+    // ITX_FORCE_TILE): they only pay off for long K on large tiles
+    for (int ti = 0; ti < 11; ++ti) {
       if (ti == kSkinny || ti == kSmall) continue;
       const TileCfg ts = kTiles[ti];
       std::vector<double> costs;
