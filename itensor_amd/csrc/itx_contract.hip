@@ -639,19 +639,30 @@ __global__ __launch_bounds__(SM_T) void k_reduce_tiles(const double *__restrict_
                                                        const int64_t *__restrict__ tab, double alpha,
                                                        double beta) {
   (void)B;
-  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-    const WorkUnit ru = units[u];
+  // one item = one tile x one slice of SM_T of its SM_M * SM_N elements;
+  // four independent partial sums keep several loads in flight
+  constexpr int PARTS = SM_M * SM_N / SM_T;
+  for (int w = blockIdx.x; w < nunits * PARTS; w += gridDim.x) {
+    const WorkUnit ru = units[w / PARTS];
     const GemmTask t0 = tasks[ru.tbeg];
-    for (int e = threadIdx.x; e < SM_M * SM_N; e += SM_T) {
-      const int n = e / SM_M, m = e - n * SM_M;
-      if (ru.m0 + m >= t0.M || ru.n0 + n >= t0.N) continue;
-      double s = 0;
-      for (int q = ru.tend; q < ru.slot; ++q) s += A[size_t(q) * (SM_M * SM_N) + e];
-      double *cp = C + t0.c_off + tab[t0.tCM + ru.m0 + m] + tab[t0.tCN + ru.n0 + n];
-      double v = alpha * s;
-      if (beta != 0.0) v += beta * (*cp);
-      *cp = v;
+    const int e = (w % PARTS) * SM_T + threadIdx.x;
+    const int n = e / SM_M, m = e - n * SM_M;
+    if (ru.m0 + m >= t0.M || ru.n0 + n >= t0.N) continue;
+    const double *src = A + e;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int q = ru.tend;
+    for (; q + 4 <= ru.slot; q += 4) {
+      s0 += src[size_t(q) * (SM_M * SM_N)];
+      s1 += src[size_t(q + 1) * (SM_M * SM_N)];
+      s2 += src[size_t(q + 2) * (SM_M * SM_N)];
+      s3 += src[size_t(q + 3) * (SM_M * SM_N)];
     }
+    for (; q < ru.slot; ++q) s0 += src[size_t(q) * (SM_M * SM_N)];
+    const double s = (s0 + s1) + (s2 + s3);
+    double *cp = C + t0.c_off + tab[t0.tCM + ru.m0 + m] + tab[t0.tCN + ru.n0 + n];
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * (*cp);
+    *cp = v;
   }
 }
 
@@ -1074,7 +1085,9 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
       const double c = std::max(1.0, std::ceil(total / target));
       split = maxw > 2.0 * c && total * (SM_M * SM_N) * 8.0 / c < double(size_t(1) << 30);
       if (split) {
-        const int per = int(c);
+        // at least two k-chunks per unit: a one-chunk unit writes a
+        // partial tile for as many bytes as it reads
+        const int per = std::max(2, int(c));
         for (auto &g : groups) {
           const int vb = int(vt.size());
           for (int t = g.tb; t < g.te; ++t)
@@ -1139,7 +1152,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->nred = int(red.size());
     if (p->nred) {
       p->fn_red = reinterpret_cast<const void *>(&k_reduce_tiles);
-      p->grid_red = int(std::min<int64_t>(int64_t(red.size()), int64_t(cus) * 8));
+      p->grid_red = int(std::min<int64_t>(int64_t(red.size()) * (SM_M * SM_N / SM_T), int64_t(cus) * 8));
     }
     if (std::getenv("ITX_PLAN_LOG"))
       std::fprintf(stderr, "[itx plan] tasks=%d units=%d tile=%dx%d%s grid=%d akc=%d bkc=%d flops=%.3e split=%d\n",

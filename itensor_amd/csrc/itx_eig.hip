@@ -8,11 +8,16 @@
 // tridiagonal form T = Q^T A Q with explicit accumulation of Q, then the
 // implicit QL iteration with Wilkinson-type shift on T (Bowdler, Martin,
 // Reinsch & Wilkinson, Numer. Math. 11 (1968) 293, "tql2"), whose plane
-// rotations are applied to Q.  Thread 0 runs the scalar QL recurrence of
-// one sweep and records the rotations in LDS; all threads then apply them
-// to their rows of Q.  Batched: block b of the grid solves matrix b;
-// matrices up to 128 stay in LDS for the whole solve.
+// rotations are applied to Q.  The work is split the way hybrid symmetric
+// eigensolvers split it: the O(n^3) parts on the device (reduction and Q in
+// one launch, one workgroup per matrix, matrices up to 128 resident in
+// LDS; the rotations applied to Q's rows in parallel in a second launch),
+// the O(n^2) scalar QL recurrence -- a chain of dependent flops that one
+// host core runs far faster than one GPU lane -- on the host in between.
 #include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <algorithm>
 #include <cstring>
 
 #include "itx_internal.h"
@@ -20,159 +25,16 @@
 namespace itx {
 namespace {
 
-constexpr int ET = 512;       // threads per matrix
+constexpr int ET = 512;       // threads per matrix (reduction above 128)
 constexpr int EMAX = 2048;    // largest n (d, e held in LDS)
-constexpr int ROTG = 6144;    // rotation buffer of the global-memory kernel
 constexpr int QL_MAXIT = 60;  // QL sweeps per eigenvalue before giving up
 
 struct EigTask {
   int32_t n, pad;
-  int64_t a_off, w_off, z_off;  // A (in/out, col-major), w, workspace Z (n*n, row-major)
+  int64_t a_off, w_off;  // A (in/out, col-major), w
+  int64_t z_off;         // workspace: Q, column-major n x (n+1) (the last column a dummy)
+  int64_t de_off;        // workspace: (d, e) handed to the host
 };
-
-// Givens pair of (f, g): r = sqrt(f^2 + g^2), s = f / r, c = g / r, with
-// one reciprocal square root instead of hypot and two divisions (the
-// operands of tql2's recurrence are bounded by the matrix norm, so the
-// unscaled sum of squares cannot overflow).
-__device__ __forceinline__ void givens(double f, double g, double &r, double &s, double &c) {
-  const double t = fma(f, f, g * g);
-  if (t == 0.0) {
-    r = 0.0;
-    s = 0.0;
-    c = 1.0;
-    return;
-  }
-  const double ri = rsqrt(t);
-  r = t * ri;
-  s = f * ri;
-  c = g * ri;
-}
-
-// One implicit QL sweep of tql2 on (d, e)[l..m].  Executed by all lanes of
-// wave 0 with identical values (uniform control flow; LDS stores of equal
-// values); the plane rotations are appended to (rc, rs, ri).  Operands of
-// the next step are loaded before the current step's dependency chain.
-// Returns the number of rotations (negative - 1 on an early split).
-__device__ int ql_sweep(double *d, double *e, int l, int m, double *rc, double *rs, int *ri) {
-  double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-  double r = sqrt(fma(g, g, 1.0));
-  g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-  double s = 1, c = 1, p = 0;
-  int nrot = 0;
-  double ei = e[m - 1], di = d[m - 1], di1 = d[m];
-  for (int i = m - 1; i >= l; --i) {
-    const int in = i > 0 ? i - 1 : 0;
-    const double ei_next = e[in], di_next = d[in];
-    const double f = s * ei, b = c * ei;
-    double cs, sn;
-    givens(f, g, r, sn, cs);
-    e[i + 1] = r;
-    if (r == 0) {
-      d[i + 1] = di1 - p;
-      e[m] = 0;
-      return -(nrot + 1);  // early split: d[l], e[l] stay
-    }
-    s = sn;
-    c = cs;
-    g = di1 - p;
-    r = (di - g) * s + 2.0 * c * b;
-    p = s * r;
-    d[i + 1] = g + p;
-    g = c * r - b;
-    rc[nrot] = c;
-    rs[nrot] = s;
-    ri[nrot] = i;
-    ++nrot;
-    di1 = di;
-    di = di_next;
-    ei = ei_next;
-  }
-  d[l] -= p;
-  e[l] = g;
-  e[m] = 0;
-  return nrot;
-}
-
-// Implicit QL on the tridiagonal (d, e) with the deflation test of tql2
-// (|e_m| against the largest |d|+|e| seen so far, so eigenvalues near zero
-// converge to absolute accuracy), rotations applied to the columns of Z
-// (element (r, c) at Z[r * zr + c * zc]).  Wave 0 runs the scalar
-// recurrence -- it never depends on Z -- and fills the rotation buffer
-// (cap entries) sweep by sweep; then every thread applies the buffered
-// rotations to its rows of Z, carrying the updated element of one
-// rotation into the next one in a register.  Returns 1 when an eigenvalue
-// needs more than QL_MAXIT sweeps.
-template <int NT>
-__device__ int ql_phase(int n, double *d, double *e, double *rc, double *rs, int *ri, int cap, int *sh,
-                        double *Z, int zr, int zc) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // wave 0's recurrence state (uniform across its lanes)
-  int l = 0, iter = 0;
-  bool newl = true;
-  double tst1 = 0;
-  while (true) {
-    if (wave == 0) {
-      int cnt = 0, done = 0, fail = 0;
-      while (cnt + n <= cap) {
-        if (l >= n) {
-          done = 1;
-          break;
-        }
-        if (newl) {
-          tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
-          iter = 0;
-          newl = false;
-        }
-        int m = n - 1;
-        for (int base = l; base < n - 1; base += 64) {
-          const int mm = base + lane;
-          const bool neg = mm < n - 1 && fabs(e[mm]) <= DBL_EPSILON * fmax(fabs(d[mm]) + fabs(d[mm + 1]), tst1);
-          const unsigned long long bal = __ballot(neg);
-          if (bal) {
-            m = base + __ffsll(bal) - 1;
-            break;
-          }
-        }
-        if (m == l) {
-          ++l;
-          newl = true;
-          continue;
-        }
-        if (iter >= QL_MAXIT) {
-          fail = 1;
-          break;
-        }
-        const int nr = ql_sweep(d, e, l, m, rc + cnt, rs + cnt, ri + cnt);
-        cnt += nr < 0 ? -nr - 1 : nr;
-        ++iter;
-      }
-      if (lane == 0) {
-        sh[0] = cnt;
-        sh[1] = done;
-        sh[2] = fail;
-      }
-    }
-    __syncthreads();
-    const int cnt = sh[0], done = sh[1], fail = sh[2];
-    for (int r = tid; r < n; r += NT) {
-      double *zrow = Z + size_t(r) * zr;
-      int prev = -2;
-      double carry = 0;
-      for (int q = 0; q < cnt; ++q) {
-        const int i = ri[q];
-        const double c = rc[q], sn = rs[q];
-        const double f = (i + 1 == prev) ? carry : zrow[size_t(i + 1) * zc];
-        const double z = zrow[size_t(i) * zc];
-        zrow[size_t(i + 1) * zc] = sn * z + c * f;
-        carry = c * z - sn * f;
-        zrow[size_t(i) * zc] = carry;
-        prev = i;
-      }
-    }
-    __syncthreads();  // the buffer and sh are read before wave 0 rewrites them
-    if (done || fail) return fail;
-  }
-}
 
 __device__ __forceinline__ double bsum(double v, double *red) {
 #pragma unroll
@@ -186,14 +48,12 @@ __device__ __forceinline__ double bsum(double v, double *red) {
   return s;
 }
 
+// Reduction and Q for n > 128: A in global memory (overwritten by the
+// reflectors), W as scratch for p, Q into the workspace.
 __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
-                                             double *__restrict__ Wbase, double *__restrict__ Zbase,
-                                             int *__restrict__ status) {
-  __shared__ double d[EMAX], e[EMAX];
-  __shared__ double rc[ROTG], rs[ROTG];
-  __shared__ int ri[ROTG];
+                                             double *__restrict__ Wbase, double *__restrict__ Zbase) {
+  __shared__ double d[EMAX], e[EMAX], tau_[EMAX];
   __shared__ double red[ET / 64];
-  __shared__ int sh[4];
 
   const EigTask tk = tasks[blockIdx.x];
   const int n = tk.n;
@@ -222,7 +82,7 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
     }
     if (tid == 0) {
       e[k] = alpha;
-      rc[k] = tau;  // rc doubles as tau storage during this phase
+      tau_[k] = tau;
     }
     if (tau == 0) {
       __syncthreads();
@@ -257,100 +117,105 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
   }
   __syncthreads();
 
-  // ---- 2. Q = H_0 H_1 ... H_{n-3}, accumulated backwards, Z row-major ----
+  // ---- 2. Q = H_0 H_1 ... H_{n-3}, accumulated backwards, Z column-major ----
+  // (one wave per column: the lanes read and update it coalesced)
   for (size_t x = tid; x < size_t(n) * n; x += ET) Z[x] = ((x / n) == (x % n)) ? 1.0 : 0.0;
   __syncthreads();
-  for (int k = n - 3; k >= 0; --k) {
-    const double tau = rc[k];
-    if (tau == 0) continue;
-    const int L = n - k - 1;
-    const double *v = A + size_t(k) * n + (k + 1);
-    // Z_sub -= tau v (v^T Z_sub); columns c handled by threads
-    for (int c = tid; c < L; c += ET) {
-      double sc = 0;
-      for (int i = 0; i < L; ++i) sc += v[i] * Z[size_t(k + 1 + i) * n + (k + 1 + c)];
-      sc *= tau;
-      for (int i = 0; i < L; ++i) Z[size_t(k + 1 + i) * n + (k + 1 + c)] -= sc * v[i];
+  {
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int k = n - 3; k >= 0; --k) {
+      const double tau = tau_[k];
+      if (tau == 0) continue;
+      const int L = n - k - 1;
+      const double *v = A + size_t(k) * n + (k + 1);
+      // Z_sub -= tau v (v^T Z_sub)
+      for (int c = wv; c < L; c += ET / 64) {
+        double *zc = Z + size_t(k + 1 + c) * n + (k + 1);
+        double sc = 0;
+        for (int i = lane; i < L; i += 64) sc += v[i] * zc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sc += __shfl_xor(sc, o, 64);
+        sc *= tau;
+        for (int i = lane; i < L; i += 64) zc[i] -= sc * v[i];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
-  // ---- 3. implicit QL on (d, e), rotations applied to Z's columns ----
-  if (ql_phase<ET>(n, d, e, rc, rs, ri, ROTG, sh, Z, n, 1)) {
-    if (tid == 0) status[blockIdx.x] = 1;
-    return;
+  // ---- 3. (d, e) to the host, which runs the QL recurrence ----
+  double *DE = Zbase + tk.de_off;
+  for (int i = tid; i < n; i += ET) {
+    DE[i] = d[i];
+    DE[n + i] = e[i];
   }
-
-  // ---- 4. sort descending (stable rank), write eigenpairs ----
-  for (int c = tid; c < n; c += ET) {
-    const double dc = d[c];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) rank += (d[j] > dc) || (d[j] == dc && j < c);
-    ri[c] = rank;
-  }
-  __syncthreads();
-  for (int c = tid; c < n; c += ET) W[ri[c]] = d[c];
-  for (size_t x = tid; x < size_t(n) * n; x += ET) {
-    const size_t r = x / n, c = x % n;
-    A[size_t(ri[c]) * n + r] = Z[x];
-  }
-  if (tid == 0) status[blockIdx.x] = 0;
 }
 
-// ---- LDS-resident form for n <= NMAX (<= 128) ----------------------
-// The matrix lives in LDS for the whole solve: Householder reduction, Q
-// formed in place from the stored reflectors (backward accumulation, the
-// row and column of each new trailing index reset to the unit vector), the
-// QL sweeps applied to Q's columns, then the sorted eigenpairs written out.
+// ---- LDS-resident reduction for n <= NMAX (<= 128) -------------------
+// The matrix lives in LDS: Householder reduction, then Q formed in place
+// from the stored reflectors (backward accumulation, the row and column of
+// each new trailing index reset to the unit vector), written out with
+// (d, e) for the host.
 constexpr int ES = 256;
 
-__device__ __forceinline__ double bsum_s(double v, double *red) {
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double s = 0;
-  for (int i = 0; i < ES / 64; ++i) s += red[i];
-  return s;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-template <int NMAX, int ROT>
-__global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
-                                                 double *__restrict__ Wbase, int *__restrict__ status) {
-  static_assert(ROT >= 2 * NMAX, "rotation buffer below two sweeps");
+template <int NMAX>
+__global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tasks, const double *__restrict__ Abase,
+                                                 double *__restrict__ Zbase) {
   constexpr int LD = NMAX + 1;
   __shared__ double S[NMAX * LD];
   __shared__ double d[NMAX], e[NMAX], tau[NMAX], pv[NMAX];
-  __shared__ double rc[ROT], rs[ROT];
-  __shared__ int ri[ROT];
-  __shared__ double red[ES / 64];
-  __shared__ int sh[4];
+  __shared__ double part[(ES / 64) * NMAX];
 
   const EigTask tk = tasks[blockIdx.x];
   const int n = tk.n;
   const int tid = threadIdx.x;
-  double *__restrict__ A = Abase + tk.a_off;
-  double *__restrict__ W = Wbase + tk.w_off;
-  if (n <= 0 || n > NMAX) {
-    if (tid == 0) status[blockIdx.x] = (n <= 0) ? 0 : 2;
-    return;
-  }
-  for (int x = tid; x < n * n; x += ES) {
-    const int c = x / n, r = x - c * n;
-    S[c * LD + r] = A[x];
+  const double *__restrict__ A = Abase + tk.a_off;
+  if (n <= 0 || n > NMAX) return;  // the host sorts matrices into size classes
+  // staged through registers, eight loads in flight per thread
+  for (int x0 = 0; x0 < n * n; x0 += 8 * ES) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int x = x0 + u * ES + tid;
+      v[u] = x < n * n ? A[x] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int x = x0 + u * ES + tid;
+      if (x < n * n) {
+        const int c = x / n, r = x - c * n;
+        S[c * LD + r] = v[u];
+      }
+    }
   }
   __syncthreads();
 
   // ---- 1. Householder tridiagonalisation ----
+  // Every wave computes the reflector redundantly (lanes over rows, shuffle
+  // sums), the product p = tau S v is split over the four waves by column
+  // residue mod 4, and the rank-2 update runs waves over columns, lanes
+  // over rows: three barriers per step.
+  constexpr int PL = (NMAX + 63) / 64;  // rows per lane
+  constexpr int NW = ES / 64;
+  const int lane = tid & 63, wv = tid >> 6;
   for (int k = 0; k + 2 < n; ++k) {
     const int L = n - k - 1;
-    double *col = S + k * LD + (k + 1);
-    double s = 0;
-    for (int i = 1 + tid; i < L; i += ES) s += col[i] * col[i];
-    const double sigma = bsum_s(s, red);
-    const double x0 = col[0];
+    double *col = S + k * LD + (k + 1);  // x = S(k+1:, k)
+    double xr[PL];
+    double sg = 0;
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const int i = lane + 64 * u;
+      xr[u] = i < L ? col[i] : 0.0;
+      if (i >= 1) sg += xr[u] * xr[u];
+    }
+    const double sigma = wave_sum(sg);
+    const double x0 = __shfl(xr[0], 0, 64);
     double t = 0, alpha = x0, v0 = x0;
     if (sigma > 0) {
       const double mu = sqrt(x0 * x0 + sigma);
@@ -358,29 +223,70 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
       alpha = (x0 <= 0) ? mu : -mu;
       t = 2.0 / (v0 * v0 + sigma);
     }
-    __syncthreads();  // col[0] read by all before it becomes v0
-    if (tid == 0) {
-      col[0] = v0;
-      e[k] = alpha;
-      tau[k] = t;
+    if (lane == 0) xr[0] = v0;  // this lane's copy of v
+    if (t == 0) {
+      if (tid == 0) {
+        e[k] = alpha;
+        tau[k] = 0;
+      }
+      continue;  // S unchanged; nothing to order
+    }
+    // partial products over columns j = wv mod NW
+    double acc[PL];
+#pragma unroll
+    for (int u = 0; u < PL; ++u) acc[u] = 0;
+    for (int j = wv; j < L; j += NW) {
+      const double vj = (j == 0) ? v0 : col[j];
+      const double *Sj = S + (k + 1 + j) * LD + (k + 1);
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < L) acc[u] += Sj[i] * vj;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const int i = lane + 64 * u;
+      if (i < L) part[wv * NMAX + i] = acc[u];
     }
     __syncthreads();
-    if (t == 0) continue;
-    for (int i = tid; i < L; i += ES) {
-      double acc = 0;
-      const double *Si = S + (k + 1) * LD + (k + 1 + i);
-      for (int j = 0; j < L; ++j) acc += Si[j * LD] * col[j];
-      pv[i] = t * acc;
+    double pr[PL], dv = 0;
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const int i = lane + 64 * u;
+      double pi = 0;
+      if (i < L) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) pi += part[q * NMAX + i];
+      }
+      pr[u] = t * pi;
+      dv += xr[u] * pr[u];
+    }
+    const double Kc = 0.5 * t * wave_sum(dv);
+#pragma unroll
+    for (int u = 0; u < PL; ++u) pr[u] -= Kc * xr[u];  // w = p - K v
+    if (wv == 0) {
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < L) pv[i] = pr[u];
+      }
+      if (lane == 0) {
+        col[0] = v0;  // col now holds v
+        e[k] = alpha;
+        tau[k] = t;
+      }
     }
     __syncthreads();
-    s = 0;
-    for (int i = tid; i < L; i += ES) s += col[i] * pv[i];
-    const double Kc = 0.5 * t * bsum_s(s, red);
-    for (int i = tid; i < L; i += ES) pv[i] -= Kc * col[i];
-    __syncthreads();
-    for (int x = tid; x < L * L; x += ES) {
-      const int j = x / L, i = x - j * L;
-      S[(k + 1 + j) * LD + (k + 1 + i)] -= col[i] * pv[j] + pv[i] * col[j];
+    // S -= v w^T + w v^T
+    for (int j = wv; j < L; j += NW) {
+      const double vj = col[j], wj = pv[j];
+      double *Sj = S + (k + 1 + j) * LD + (k + 1);
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < L) Sj[i] -= xr[u] * wj + pr[u] * vj;
+      }
     }
     __syncthreads();
   }
@@ -393,6 +299,8 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
   __syncthreads();
 
   // ---- 2. Q = H_0 ... H_{n-3} in place (trailing block grows backwards) ----
+  // s_c = tau v^T Z(:, c) split over the waves by row residue mod 4 (lanes
+  // over columns), then Z -= v s^T with waves over columns, lanes over rows.
   for (int k = n - 3; k >= 0; --k) {
     const int j0 = k + 1;
     for (int i = j0 + tid; i < n; i += ES) {
@@ -401,15 +309,42 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
     }
     __syncthreads();
     const double t = tau[k];
-    if (t != 0) {
-      const double *v = S + k * LD + j0;  // v_k, first entry at row j0
-      const int L = n - j0;
-      for (int c = tid; c < L; c += ES) {
-        double *Tc = S + (j0 + c) * LD + j0;
-        double sc = 0;
-        for (int i = 0; i < L; ++i) sc += v[i] * Tc[i];
-        sc *= t;
-        for (int i = 0; i < L; ++i) Tc[i] -= sc * v[i];
+    if (t == 0) continue;  // block-uniform; the next step's reset is ordered by its own barrier
+    const double *v = S + k * LD + j0;  // v_k, first entry at row j0
+    const int L = n - j0;
+    double acc[PL];
+#pragma unroll
+    for (int u = 0; u < PL; ++u) acc[u] = 0;
+    for (int i = wv; i < L; i += NW) {
+      const double vi = v[i];
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const int c = lane + 64 * u;
+        if (c < L) acc[u] += vi * S[(j0 + c) * LD + j0 + i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const int c = lane + 64 * u;
+      if (c < L) part[wv * NMAX + c] = acc[u];
+    }
+    __syncthreads();
+    double vr[PL];
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const int i = lane + 64 * u;
+      vr[u] = i < L ? v[i] : 0.0;
+    }
+    for (int c = wv; c < L; c += NW) {
+      double sc = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) sc += part[q * NMAX + c];
+      sc *= t;
+      double *Tc = S + (j0 + c) * LD + j0;
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < L) Tc[i] -= sc * vr[u];
       }
     }
     __syncthreads();
@@ -420,26 +355,212 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
   }
   __syncthreads();
 
-  // ---- 3. implicit QL (tql2 deflation), rotations on Q's columns ----
-  if (ql_phase<ES>(n, d, e, rc, rs, ri, ROT, sh, S, 1, LD)) {
-    if (tid == 0) status[blockIdx.x] = 1;
-    return;
-  }
-
-  // ---- 4. sort descending (stable rank), write eigenpairs ----
-  for (int c = tid; c < n; c += ES) {
-    const double dc = d[c];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) rank += (d[j] > dc) || (d[j] == dc && j < c);
-    ri[c] = rank;
-  }
-  __syncthreads();
-  for (int c = tid; c < n; c += ES) W[ri[c]] = d[c];
+  // ---- 3. Q (column-major) and (d, e) out; the host runs the QL recurrence ----
+  double *Z = Zbase + tk.z_off;
   for (int x = tid; x < n * n; x += ES) {
     const int c = x / n, r = x - c * n;
-    A[ri[c] * n + r] = S[c * LD + r];
+    Z[x] = S[c * LD + r];
   }
-  if (tid == 0) status[blockIdx.x] = 0;
+  double *DE = Zbase + tk.de_off;
+  for (int i = tid; i < n; i += ES) {
+    DE[i] = d[i];
+    DE[n + i] = e[i];
+  }
+}
+
+// ---- hybrid form: rotations recorded by the host, applied here ------------
+// Every QL sweep is a chain of rotations on consecutive index pairs
+// (i, i+1), i = top, top-1, ..., top-cnt+1.  The host sends one stream of
+// double pairs per matrix: a header (top, cnt) per sweep followed by that
+// sweep's (c, s) pairs in application order.
+struct RotMeta {
+  int64_t rot_off, ev_off;      // the stream; eigenvalues then ranks
+  int64_t a_off, w_off, z_off;  // A, w; Q (column-major, n x (n+1)) in the workspace
+  int32_t n, nrot;              // nrot: stream length (headers included)
+};
+constexpr int AT = 64;    // rows per apply block (one wave, one row per lane)
+constexpr int AK = 16;    // rotations per straight-line chunk
+constexpr int RB = 1024;  // rotations staged in LDS per batch
+
+// Applies the rotation stream to this lane's row of Q (element (row, i) at
+// zc[i * ld]; column n is a dummy the padded tail of a chunk writes to).
+// Within a sweep position i is read once, before the rotation on (i, i+1),
+// and the rotation on (i+1, i+2) has left only i+1 changed, so a chunk
+// loads its AK positions together ahead of its rotations; the updated
+// element i travels to the next rotation in a register and each rotation
+// stores position i+1, which the chain does not touch again.
+__device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const RotMeta &mt,
+                                             const double2 *__restrict__ R, double2 *Rs) {
+  int segrem = 0, i = 0;
+  double carry = 0;
+  for (int b0 = 0; b0 < mt.nrot; b0 += RB) {
+    const int qn = min(RB, mt.nrot - b0);
+    double2 ld_[RB / AT];  // all loads of the batch in flight at once
+#pragma unroll
+    for (int u = 0; u < RB / AT; ++u) {
+      const int x = u * AT + threadIdx.x;
+      ld_[u] = x < qn ? R[b0 + x] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < RB / AT; ++u) Rs[u * AT + threadIdx.x] = ld_[u];
+    __syncthreads();
+    int q = 0;
+    while (q < qn) {
+      if (segrem == 0) {  // a sweep header
+        const double2 h = Rs[q++];
+        i = int(h.x);
+        segrem = int(h.y);
+        carry = zc[size_t(i + 1) * ld];
+        continue;  // the header may end the batch
+      }
+      const int kk = min(AK, min(segrem, qn - q));
+      // all loads of the chunk ahead of its stores (the stores could alias
+      // the rotation buffer as far as the compiler knows)
+      double z[AK];
+      double2 g[AK];
+#pragma unroll
+      for (int t = 0; t < AK; ++t) {
+        z[t] = zc[size_t(max(i - t, 0)) * ld];
+        // past kk the pair (c, s) = (0, -1) passes the carry through
+        // exactly and its store goes to the dummy column, so the chain
+        // from one carry to the next is a single FMA
+        const double2 gl = Rs[q + t];  // in the buffer's AK-entry tail when past qn
+        g[t] = make_double2(t < kk ? gl.x : 0.0, t < kk ? gl.y : -1.0);
+      }
+#pragma unroll
+      for (int t = 0; t < AK; ++t) {
+        const bool on = t < kk;
+        const double cz = g[t].x * z[t];
+        const double up = fma(g[t].y, z[t], g[t].x * carry);
+        carry = fma(-g[t].y, carry, cz);
+        zc[size_t(on ? i - t + 1 : n) * ld] = up;
+      }
+      q += kk;
+      segrem -= kk;
+      i -= kk;
+      if (segrem == 0) zc[size_t(i + 1) * ld] = carry;
+    }
+  }
+}
+
+// Q <- Q G_1 G_2 ..., then this block's rows of the sorted eigenvectors
+// (column rank[c] of A is column c of Q) and, from the first block of a
+// matrix, the sorted eigenvalues.  LC > 0: the block's 64 rows of Q are
+// staged in LDS (n <= LC); LC == 0: they are updated in the workspace.
+template <int LC>
+__global__ __launch_bounds__(AT) void k_syev_apply(const RotMeta *__restrict__ meta,
+                                                   const double2 *__restrict__ rots, const double *__restrict__ evr, double *__restrict__ Abase,
+                                                   double *__restrict__ Wbase, double *__restrict__ Zbase) {
+  __shared__ double2 Rs[RB + AK];
+  __shared__ double Zs[LC > 0 ? (LC + 1) * AT : 1];
+  const RotMeta mt = meta[blockIdx.x];
+  const int n = mt.n;
+  const int r0 = blockIdx.y * AT;
+  if (r0 >= n) return;
+  const int lane = threadIdx.x;
+  const int r = min(r0 + lane, n - 1);  // lanes past the last row redo row n-1 (same values)
+  const bool act = r0 + lane < n;
+  const double *Zg = Zbase + mt.z_off + r;
+  const double2 *R = rots + mt.rot_off;
+  double *zc;
+  int ld;
+  if constexpr (LC > 0) {
+    zc = Zs + lane;
+    ld = AT;
+    for (int c0 = 0; c0 < n; c0 += 16) {  // sixteen loads in flight
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = c0 + u < n ? Zg[size_t(c0 + u) * n] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (c0 + u < n) zc[(c0 + u) * AT] = v[u];
+    }
+  } else {
+    zc = Zbase + mt.z_off + r;
+    ld = n;
+  }
+  apply_stream(zc, ld, n, mt, R, Rs);
+  // ranks to LDS (the rotation buffer is free now), then the sorted output
+  const double *ev = evr + mt.ev_off, *rk = ev + n;
+  int *rks = reinterpret_cast<int *>(Rs);
+  __syncthreads();
+  for (int c = lane; c < n; c += AT) {
+    const int k = int(rk[c]);
+    rks[c] = k;
+    if (blockIdx.y == 0) Wbase[mt.w_off + k] = ev[c];
+  }
+  __syncthreads();
+  if (!act) return;
+  double *__restrict__ A = Abase + mt.a_off + r;
+  for (int c = 0; c < n; ++c) A[size_t(rks[c]) * n] = zc[size_t(c) * ld];
+}
+
+// sqrt(a^2 + b^2) without overflow or underflow of the squares
+inline double pyth(double a, double b) {
+  a = std::fabs(a);
+  b = std::fabs(b);
+  const double w = std::max(a, b), v = std::min(a, b);
+  if (w == 0) return 0;
+  const double q = v / w;
+  return w * std::sqrt(1.0 + q * q);
+}
+
+// Implicit QL with the tql2 deflation test on the host (the recurrence is
+// a serial chain of a few dependent flops per rotation, which one CPU core
+// runs an order of magnitude faster than one GPU lane).  Appends a header
+// (top, cnt) and the (c, s) pairs of every sweep to the stream; d ends up
+// holding the eigenvalues.
+bool host_ql(int n, double *d, double *e, std::vector<double2> &rots) {
+  double tst1 = 0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    tst1 = std::max(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+    while (true) {
+      int m = l;
+      for (; m < n - 1; ++m)
+        if (std::fabs(e[m]) <= DBL_EPSILON * std::max(std::fabs(d[m]) + std::fabs(d[m + 1]), tst1)) break;
+      if (m == l) break;
+      if (iter++ >= QL_MAXIT) return false;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = pyth(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+      double s = 1, c = 1, p = 0;
+      bool early = false;
+      const size_t hdr = rots.size();
+      rots.push_back(make_double2(0.0, 0.0));
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i], b = c * e[i];
+        r = pyth(f, g);
+        e[i + 1] = r;
+        if (r == 0) {
+          d[i + 1] -= p;
+          e[m] = 0;
+          early = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        rots.push_back(make_double2(c, s));
+      }
+      const size_t cnt = rots.size() - hdr - 1;
+      if (cnt)
+        rots[hdr] = make_double2(double(m - 1), double(cnt));
+      else
+        rots.pop_back();
+      if (!early) {
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0;
+      }
+    }
+  }
+  return true;
 }
 
 }  // namespace
@@ -447,10 +568,13 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
 int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, double *Abase,
                  const int64_t *woffs, double *Wbase) {
   if (nmat <= 0) return ITX_OK;
-  // three size classes: LDS-resident up to 64 and up to 128, else the
-  // global-memory kernel (with its n x n workspace per matrix)
+  // The kernels reduce every matrix to tridiagonal form and build Q; the
+  // host runs the QL recurrences; k_syev_apply applies the recorded
+  // rotations and writes the sorted eigenpairs.  Reduction size classes:
+  // LDS-resident up to 64 and up to 128, else the global-memory kernel.
+  // Workspace: every (d, e) first (one copy to the host), then every Q.
   std::vector<EigTask> t64, t128, tg;
-  int64_t zsize = 0;
+  int64_t desize = 0, qsize = 0;
   for (int b = 0; b < nmat; ++b) {
     if (ns[b] < 0 || ns[b] > EMAX) return set_err(ctx, ITX_ERR_UNSUPPORTED, "itx_syev: n above 2048");
     if (ns[b] == 0) continue;
@@ -458,15 +582,16 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
     t.n = ns[b];
     t.a_off = aoffs[b];
     t.w_off = woffs[b];
-    if (ns[b] <= 64) {
+    t.de_off = desize;
+    desize += 2 * int64_t(ns[b]);
+    t.z_off = qsize;
+    qsize += int64_t(ns[b]) * (ns[b] + 1);
+    if (ns[b] <= 64)
       t64.push_back(t);
-    } else if (ns[b] <= 128) {
+    else if (ns[b] <= 128)
       t128.push_back(t);
-    } else {
-      t.z_off = zsize;
-      zsize += int64_t(ns[b]) * ns[b];
+    else
       tg.push_back(t);
-    }
   }
   const int ntask = int(t64.size() + t128.size() + tg.size());
   if (ntask == 0) return ITX_OK;
@@ -474,9 +599,10 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   all.insert(all.end(), t64.begin(), t64.end());
   all.insert(all.end(), t128.begin(), t128.end());
   all.insert(all.end(), tg.begin(), tg.end());
+  for (auto &t : all) t.z_off += desize;
+  const int64_t zsize = desize + qsize;
   const size_t tbytes = sizeof(EigTask) * size_t(ntask);
-  const size_t sbytes = sizeof(int) * size_t(ntask);
-  const size_t zoff = (tbytes + sbytes + 255) & ~size_t(255);
+  const size_t zoff = (tbytes + 255) & ~size_t(255);
   void *ws = nullptr;
   int rc = pool_alloc(ctx, zoff + sizeof(double) * size_t(zsize), &ws);
   if (rc) return rc;
@@ -487,26 +613,88 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
     return rc;
   }
   const EigTask *dt = reinterpret_cast<const EigTask *>(wsb);
-  int *dst = reinterpret_cast<int *>(wsb + tbytes);
+  double *Z = reinterpret_cast<double *>(wsb + zoff);
   const int n64 = int(t64.size()), n128 = int(t128.size()), ng = int(tg.size());
-  if (n64)
-    hipLaunchKernelGGL((k_syev_lds<64, 4096>), dim3(n64), dim3(ES), 0, ctx->stream, dt, Abase, Wbase, dst);
-  if (n128)
-    hipLaunchKernelGGL((k_syev_lds<128, 1024>), dim3(n128), dim3(ES), 0, ctx->stream, dt + n64, Abase, Wbase, dst + n64);
-  if (ng)
-    hipLaunchKernelGGL(k_syev, dim3(ng), dim3(ET), 0, ctx->stream, dt + n64 + n128, Abase, Wbase,
-                       reinterpret_cast<double *>(wsb + zoff), dst + n64 + n128);
+  if (n64) hipLaunchKernelGGL(k_syev_lds<64>, dim3(n64), dim3(ES), 0, ctx->stream, dt, Abase, Z);
+  if (n128) hipLaunchKernelGGL(k_syev_lds<128>, dim3(n128), dim3(ES), 0, ctx->stream, dt + n64, Abase, Z);
+  if (ng) hipLaunchKernelGGL(k_syev, dim3(ng), dim3(ET), 0, ctx->stream, dt + n64 + n128, Abase, Wbase, Z);
   hipError_t e = hipGetLastError();
-  std::vector<int> st(size_t(ntask), 0);
-  if (e == hipSuccess) e = hipMemcpyAsync(st.data(), dst, sbytes, hipMemcpyDeviceToHost, ctx->stream);
+  // the tridiagonals of every matrix to the host (one wait)
+  std::vector<double> de(static_cast<size_t>(desize));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(de.data(), Z, sizeof(double) * size_t(desize), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  pool_free(ctx, ws);
-  if (e != hipSuccess) return hip_err(ctx, e, "itx_syev");
-  for (int b = 0; b < ntask; ++b) {
-    if (st[size_t(b)] == 1) return set_err(ctx, ITX_ERR_NOCONV, "itx_syev: QL iteration did not converge");
-    if (st[size_t(b)] != 0) return set_err(ctx, ITX_ERR_ARG, "itx_syev: size class mismatch");
+  if (e != hipSuccess) {
+    pool_free(ctx, ws);
+    return hip_err(ctx, e, "itx_syev");
   }
-  return ITX_OK;
+  std::vector<double2> rots;
+  std::vector<double> evr;  // per matrix: eigenvalues (QL order), then ranks
+  // apply classes: Q rows staged in LDS for n <= 64, 128, 256; else global
+  std::vector<RotMeta> cls[4];
+  int maxn[4] = {0, 0, 0, 0};
+  for (int b = 0; b < ntask; ++b) {
+    const EigTask &t = all[size_t(b)];
+    const int n = t.n;
+    double *d = de.data() + t.de_off, *ee = d + n;
+    RotMeta mt{};
+    mt.rot_off = int64_t(rots.size());
+    if (!host_ql(n, d, ee, rots)) {
+      pool_free(ctx, ws);
+      return set_err(ctx, ITX_ERR_NOCONV, "itx_syev: QL iteration did not converge");
+    }
+    mt.nrot = int32_t(int64_t(rots.size()) - mt.rot_off);
+    mt.ev_off = int64_t(evr.size());
+    mt.a_off = t.a_off;
+    mt.w_off = t.w_off;
+    mt.z_off = t.z_off;
+    mt.n = n;
+    evr.insert(evr.end(), d, d + n);
+    for (int c = 0; c < n; ++c) {  // stable descending rank
+      int rank = 0;
+      for (int j = 0; j < n; ++j) rank += (d[j] > d[c]) || (d[j] == d[c] && j < c);
+      evr.push_back(double(rank));
+    }
+    const int k = n <= 64 ? 0 : n <= 128 ? 1 : n <= 256 ? 2 : 3;
+    cls[k].push_back(mt);
+    maxn[k] = std::max(maxn[k], n);
+  }
+  std::vector<RotMeta> meta;
+  for (auto &c : cls) meta.insert(meta.end(), c.begin(), c.end());
+  const size_t mb = sizeof(RotMeta) * meta.size(), rb = sizeof(double2) * rots.size(),
+               eb = sizeof(double) * evr.size();
+  const size_t o_r = (mb + 255) & ~size_t(255), o_e = (o_r + rb + 255) & ~size_t(255);
+  void *ub = nullptr;
+  rc = pool_alloc(ctx, o_e + eb, &ub);
+  if (rc) {
+    pool_free(ctx, ws);
+    return rc;
+  }
+  char *u = static_cast<char *>(ub);
+  rc = stage_upload(ctx, u, meta.data(), mb);
+  if (!rc && rb) rc = stage_upload(ctx, u + o_r, rots.data(), rb);
+  if (!rc) rc = stage_upload(ctx, u + o_e, evr.data(), eb);
+  if (!rc) {
+    const RotMeta *dm = reinterpret_cast<const RotMeta *>(u);
+    const double2 *dr = reinterpret_cast<const double2 *>(u + o_r);
+    const double *dv = reinterpret_cast<const double *>(u + o_e);
+    using Kfn = void (*)(const RotMeta *, const double2 *, const double *, double *, double *, double *);
+    const Kfn fns[4] = {k_syev_apply<64>, k_syev_apply<128>, k_syev_apply<256>, k_syev_apply<0>};
+    size_t first = 0;
+    for (int k = 0; k < 4; ++k) {
+      const size_t cnt = cls[k].size();
+      if (cnt)
+        hipLaunchKernelGGL(fns[k], dim3(unsigned(cnt), unsigned((maxn[k] + AT - 1) / AT)), dim3(AT), 0, ctx->stream,
+                           dm + first, dr, dv, Abase, Wbase, Z);
+      first += cnt;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_err(ctx, e, "itx_syev apply");
+  }
+  // stream-ordered frees: reused only by later work on the stream
+  pool_free(ctx, ub);
+  pool_free(ctx, ws);
+  return rc;
 }
 
 }  // namespace itx
