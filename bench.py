@@ -249,6 +249,46 @@ def heisenberg_qn_chain(H, N: int):
     return mpo, A
 
 
+def hubbard_cylinder(H, Nx: int, Ny: int, t: float = 1.0, U: float = 4.0, yperiodic: bool = True):
+    """Hubbard model on the Nx x Ny square lattice (YPeriodic cylinder, site
+    order of squareLattice) with (Nf, Sz) conserved: the MPO from the host
+    layer's AutoMPO (-t c+_i,s c_j,s + h.c. on every bond, U n_up n_dn on
+    every site) and a half-filled product state (Up, Dn, Up, ...) whose link
+    QNs give every site tensor but the last flux 0; the last one carries the
+    state's total charge (Nf = N, Sz = 0)."""
+    N = Nx * Ny
+    sites = H.electron_sites(N)
+    ampo = H.AutoMPO(sites)
+    for (i, j) in H.square_lattice(Nx, Ny, yperiodic):
+        for c, a in (("Cdagup", "Cup"), ("Cdagdn", "Cdn")):
+            ampo.add(-t, c, i, a, j)
+            ampo.add(-t, c, j, a, i)
+    for i in range(1, N + 1):
+        ampo.add(U, "Nupdn", i)
+    mpo = ampo.to_mpo()
+    ids = iter(range(300001, 300001 + N + 8))
+    A, nf, sz, prev = [], 0, 0, None
+    for n in range(1, N + 1):
+        st = 1 if (n - 1) % 2 == 0 else 2  # Up, Dn
+        nf += 1
+        sz += 1 if st == 1 else -1
+        s = sites(n)
+        v = np.zeros(4)
+        v[st] = 1.0
+        if n < N:
+            q = [(k, x) for k, x in (("Nf", -nf), ("Sz", -sz)) if x]
+            link = H.Index(next(ids), 1, 1, 0, f"Link,l={n}", [(q, 1)])
+        if n == 1:
+            A.append(H.from_dense([s, link], v.reshape(4, 1)))
+        elif n == N:
+            tot = [(k, x) for k, x in (("Nf", nf), ("Sz", sz)) if x]
+            A.append(H.from_dense([prev.dag(), s], v.reshape(1, 4), tot))
+        else:
+            A.append(H.from_dense([prev.dag(), s, link], v.reshape(1, 4, 1)))
+        prev = link if n < N else None
+    return mpo, A
+
+
 DMRG_MAXDIM = [10, 20, 50, 100, 200, 400, 800, 1200, 1600, 2048]
 
 

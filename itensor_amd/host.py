@@ -50,6 +50,11 @@ set_autotune = _h.set_autotune
 LocalOp = _h.LocalOp
 random_itensor = _h.random_itensor
 from_dense = _h.from_dense
+SiteSet = _h.SiteSet
+AutoMPO = _h.AutoMPO
+spin_half_sites = _h.spin_half_sites
+electron_sites = _h.electron_sites
+square_lattice = _h.square_lattice
 HPSI_EAGER, HPSI_PLANNED, HPSI_GRAPH, HPSI_PER_BLOCK = (_h.HPSI_EAGER, _h.HPSI_PLANNED, _h.HPSI_GRAPH,
                                                         _h.HPSI_PER_BLOCK)
 

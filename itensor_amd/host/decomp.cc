@@ -498,8 +498,11 @@ std::tuple<ITensor, ITensor> factor(ITensor const &T, IndexSet const &Ais, Args 
   return {A, B};
 }
 
-// denmatDecomp, noise = 0 (decomp.h:316-418)
-Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args) {
+// denmatDecomp (decomp.h:316-418)
+Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args,
+                      DeltaRhoFn const &deltaRho) {
+  const double noise = args.getReal("Noise", 0.0);
+  if (noise > 0 && !deltaRho) throw ITError("denmatDecomp: non-zero noise needs the local Hamiltonian");
   if (!args.defined("Tags")) args.add("Tags", std::string("Link"));
   Index mid = commonIndex(A, B);
   if (dir == NoDir) dir = (mid.dir() == Out ? Fromright : Fromleft);
@@ -512,6 +515,14 @@ Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, 
   auto [cmb, ci] = combiner(cinds, args);
   ITensor AAc = cmb * AA;
   ITensor rho = AAc * dag(prime(AAc, ci));
+  if (noise > 0) {
+    // rho += noise * deltaRho, renormalised to unit trace; the trace of
+    // AAc AAc^dag is |AAc|^2 and deltaRho reports its own
+    auto [drho, trd] = deltaRho(AA, cmb, dir);
+    rho += noise * drho;
+    const double nrm = norm(AAc), tr = nrm * nrm + noise * trd;
+    if (tr > 1e-16) rho *= 1.0 / tr;
+  }
   ITensor U, D;
   auto spec = diag_hermitian(rho, U, D, args);
   cmb.dag();

@@ -374,7 +374,12 @@ std::tuple<ITensor, ITensor, ITensor> svd(ITensor const &AA, IndexSet const &Uis
 // factor (decomp.cc:538-576): T = A * B with sqrt(D) in both
 Spectrum factor(ITensor const &T, ITensor &A, ITensor &B, Args args = Args());
 std::tuple<ITensor, ITensor> factor(ITensor const &T, IndexSet const &Ais, Args const &args = Args());
-// denmatDecomp without noise (decomp.h:316-418)
-Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args);
+// denmatDecomp (decomp.h:316-418).  The noise term needs the local
+// Hamiltonian: deltaRho(AA, combiner, dir) returns the correction to the
+// density matrix and its trace (LocalOp::deltaRho, localop.h:375-406);
+// Args "Noise" > 0 without one is an error, as in the reference.
+using DeltaRhoFn = std::function<std::pair<ITensor, double>(ITensor const &, ITensor const &, Direction)>;
+Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args,
+                      DeltaRhoFn const &deltaRho = DeltaRhoFn());
 
 }  // namespace itensor

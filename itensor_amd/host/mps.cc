@@ -227,6 +227,30 @@ double LocalOp::flops(ITensor const &phi) const {
   return prg->flops;
 }
 
+std::pair<ITensor, double> LocalOp::deltaRho(ITensor const &AA, ITensor const &combine, Direction dir) const {
+  if (!Op1_ || !Op2_) throw ITError("LocalOp::deltaRho: operators not set");
+  ITensor X = AA;
+  if (dir == Fromleft) {
+    if (!LIsNull()) X *= *L_;
+    X *= *Op1_;
+  } else {
+    if (!RIsNull()) X *= *R_;
+    X *= *Op2_;
+  }
+  X = combine * noPrime(X);
+  const Index ci = commonIndex(combine, X);
+  const double nx = norm(X);
+  ITensor drho = X * dag(prime(X, ci));
+  // (drho + dag(drho with the primes of ci swapped)) / 2
+  Index c1 = ci, c2 = ci;
+  c1.prime(1);
+  c2.prime(2);
+  ITensor t = prime(prime(prime(drho, c1, 1), ci, 1), c2, -2);
+  drho += dag(t);
+  drho *= 0.5;
+  return {drho, nx * nx};
+}
+
 void LocalMPO::fuseOps(IndexSet const &phiInds) {
   if (!fuse_ || !lop_.isBulk()) return;
   auto const &H = *H_;
@@ -260,13 +284,16 @@ void LocalMPO::position(int b, MPS const &psi) {
   env_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args_in) {
+Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args_in, LocalMPO const *PH) {
   Args args = args_in;
   args.add("RespectDegenerate", args.getBool("RespectDegenerate", true));
   Index oldlink = commonIndex(psi(b), psi(b + 1));
   std::string tags = oldlink ? oldlink.tags() : std::string("Link");
   ITensor A = psi(b), B = psi(b + 1);
-  auto spec = denmatDecomp(phi, A, B, dir, args);
+  DeltaRhoFn drho;
+  if (PH && args.getReal("Noise", 0.0) > 0)
+    drho = [PH](ITensor const &AA, ITensor const &cmb, Direction d) { return PH->deltaRho(AA, cmb, d); };
+  auto spec = denmatDecomp(phi, A, B, dir, args, drho);
   if (args.getBool("DoNormalize", false)) {
     ITensor &oc = (dir == Fromleft ? B : A);
     const double nrm = norm(oc);
@@ -311,7 +338,6 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
     args.add("MaxDim", S.maxdim);
     args.add("Noise", S.noise);
     args.add("MaxIter", S.niter);
-    if (S.noise != 0) throw ITError("dmrg: noise term (deltaRho) is not part of this build");
     // precision schedule: the last FinalFP64Sweeps sweeps run the H*psi
     // contractions in FP64 whatever HpsiPrecision says
     if (long(sweeps.size() - sw) <= finalFP64) PH.setPrecision(ITX_PREC_F64);
@@ -330,7 +356,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
       DavidsonStats st;
       energy = davidson(PH, phi, args, &st);
       auto t2 = mark();
-      auto spec = svdBond(psi, b, phi, ha == 1 ? Fromleft : Fromright, args);
+      auto spec = svdBond(psi, b, phi, ha == 1 ? Fromleft : Fromright, args, &PH);
       auto t3 = mark();
       if (obs) {
         BondRecord r{int(sw + 1), b, ha, energy, spec.truncerr, commonIndex(psi(b), psi(b + 1)).dim(), st.products,

@@ -83,6 +83,11 @@ class LocalOp {
   // algorithmic flops (sum of 2mnk over every block pair) of one product
   // on a tensor with phi's structure (plans the chain if needed)
   double flops(ITensor const &phi) const;
+  // noise correction to the density matrix of the bond (localop.h:375-406):
+  // X = combine * noPrime(AA*L*Op1) (Fromleft) or noPrime(AA*R*Op2)
+  // (Fromright), drho = (X X^dag + its transpose) / 2; returns drho and its
+  // trace |X|^2
+  std::pair<ITensor, double> deltaRho(ITensor const &AA, ITensor const &combine, Direction dir) const;
 
  private:
   struct Program;
@@ -107,6 +112,9 @@ class LocalMPO {
   void setPrecision(int p) { lop_.setPrecision(p); }
   void setMode(HpsiMode m) { lop_.setMode(m); }
   double flops(ITensor const &phi) const { return lop_.flops(phi); }
+  std::pair<ITensor, double> deltaRho(ITensor const &AA, ITensor const &combine, Direction dir) const {
+    return lop_.deltaRho(AA, combine, dir);
+  }
   double envSeconds() const { return env_seconds_; }
   // pre-contract the two MPO tensors of the current bond for phi's index set
   void fuseOps(IndexSet const &phiInds);
@@ -151,7 +159,9 @@ struct BondRecord {
 };
 
 // MPS::svdBond, density-matrix route (mps/mps_impl.h:21-99)
-Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args);
+// with PH, Args "Noise" > 0 adds the noise term of PH's deltaRho
+Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args,
+                 LocalMPO const *PH = nullptr);
 
 // dmrg (mps/dmrg.h:51-76, 336-467).  obs (optional) receives one record
 // per bond step; sweep_seconds (optional) the wall time of each sweep.

@@ -4,6 +4,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include "autompo.h"
 #include "mps.h"
 
 namespace py = pybind11;
@@ -251,6 +252,12 @@ PYBIND11_MODULE(_itensor_host, m) {
         o.prepare(phi);
         return o.op.flops(phi);
       })
+      .def("delta_rho",
+           [](PyLocalOp &o, ITensor const &AA, ITensor const &combine, bool fromleft) {
+             auto r = o.op.deltaRho(AA, combine, fromleft ? Fromleft : Fromright);
+             return py::make_tuple(r.first, r.second);
+           },
+           "deltaRho(AA, combiner, dir) (localop.h:375-406): (drho, trace)")
       .def("size", [](PyLocalOp &o) { return o.op.size(); })
       .def("set_precision", [](PyLocalOp &o, int p) { o.op.setPrecision(p); })
       .def("set_mode", [](PyLocalOp &o, int m) {
@@ -272,6 +279,52 @@ PYBIND11_MODULE(_itensor_host, m) {
     return qdenseFromDense(is, v, qnFrom(div));
   }, py::arg("inds"), py::arg("array"), py::arg("div") = QNList(),
      "ITensor from a dense array (column-major); QDense of flux div when the indices carry QNs");
+  py::class_<SiteSet, std::shared_ptr<SiteSet>>(m, "SiteSet")
+      .def("__call__", [](SiteSet const &s, int i) { return s(i); })
+      .def("__len__", &SiteSet::length)
+      .def("op_matrix",
+           [](SiteSet const &s, std::string const &name, int i) {
+             SiteOp o = s.op(name, i);  // returned as M[sp, s] = <sp|O|s>
+             py::array_t<double> a({o.d, o.d});
+             auto r = a.mutable_unchecked<2>();
+             for (int sp = 0; sp < o.d; ++sp)
+               for (int x = 0; x < o.d; ++x) r(sp, x) = o.m[size_t(x + o.d * sp)];
+             return py::make_tuple(a, qnTo(o.flux), o.fermionic);
+           });
+  m.def("square_lattice", [](int Nx, int Ny, bool yperiodic) {
+    std::vector<std::pair<int, int>> r;
+    for (auto &b : squareLattice(Nx, Ny, yperiodic)) r.push_back({b.s1, b.s2});
+    return r;
+  }, py::arg("Nx"), py::arg("Ny"), py::arg("yperiodic") = false);
+  m.def("spin_half_sites", [](int N, bool qn) { return std::shared_ptr<SiteSet>(std::make_shared<SpinHalf>(N, qn)); },
+        py::arg("N"), py::arg("conserve_qns") = true);
+  m.def("electron_sites", [](int N, bool qn) { return std::shared_ptr<SiteSet>(std::make_shared<Electron>(N, qn)); },
+        py::arg("N"), py::arg("conserve_qns") = true);
+  py::class_<AutoMPO>(m, "AutoMPO")
+      .def(py::init([](std::shared_ptr<SiteSet> const &s) { return AutoMPO(*s); }))
+      .def("add", [](AutoMPO &a, double c, std::string const &o, int i) { a.add(c, o, i); })
+      .def("add", [](AutoMPO &a, double c, std::string const &o1, int i, std::string const &o2, int j) {
+        a.add(c, o1, i, o2, j);
+      })
+      .def("dense",
+           [](AutoMPO const &a) {  // host-only: W_n(s, s', h_in, h_out) arrays and link sectors
+             AutoMPO::Dense D = a.dense();
+             py::list W, links;
+             for (size_t n = 0; n < D.W.size(); ++n) {
+               const py::ssize_t d = a.sites()(int(n) + 1).dim();
+               py::array_t<double, py::array::f_style> w({d, d, py::ssize_t(D.dims[n]), py::ssize_t(D.dims[n + 1])});
+               std::copy(D.W[n].begin(), D.W[n].end(), w.mutable_data());
+               W.append(w);
+             }
+             for (auto &lb : D.links) {
+               py::list l;
+               for (auto &q : lb) l.append(py::make_tuple(qnTo(q.first), q.second));
+               links.append(l);
+             }
+             return py::make_tuple(W, D.dims, links);
+           })
+      .def("to_mpo", [](AutoMPO const &a) { return toMPO(a); });
+
   m.def("set_stream", [](uintptr_t s) { setStream(reinterpret_cast<void *>(s)); });
   m.def("synchronize", []() { check(itx_synchronize(ctx()), "itx_synchronize"); });
   m.def("set_contract_precision", &setContractPrecision);

@@ -338,3 +338,86 @@ def test_svd_degenerate_truncation_known_answers(H, args, dim):
     for TT, ind in ((A, iq), (remove_qns(A), remove_qns(A).inds[0])):
         U, D, V = H.svd(cten(TT), [cidx(ind)], args)
         assert D.inds[0].dim == dim and D.inds[1].dim == dim
+
+
+# ---- AutoMPO / Electron (config 5's model) on the device ------------------
+@pytest.mark.parametrize("Nx,Ny,U", [(3, 2, 4.0), (2, 3, 4.0), (3, 2, 0.0)])
+def test_hubbard_dmrg_matches_ed(H, Nx, Ny, U):
+    """(Nf, Sz)-conserving DMRG of the Hubbard model on small clusters (the
+    MPO from the host AutoMPO, YPeriodic where Ny > 2) against exact
+    diagonalisation of the Jordan-Wigner Hamiltonian in the half-filled
+    Sz = 0 sector; 6 sites are exact at bond dimension 64.  The first
+    sweeps carry the density-matrix noise term (as sample/dmrg.cc does),
+    which moves weight into the sectors a product start state lacks."""
+    import bench
+    from oracle import hubbard as OH
+
+    N = Nx * Ny
+    mpo, A = bench.hubbard_cylinder(H, Nx, Ny, 1.0, U, True)
+    noise = (1e-4, 1e-5, 1e-6, 1e-7) + (0,) * 10
+    sw = [{"MaxDim": m, "Cutoff": 1e-13, "Niter": 6, "Noise": z}
+          for m, z in zip((16, 32) + (64,) * 12, noise)]
+    E, recs, tens, secs = H.dmrg(mpo, A, sw, {})
+    Hed = OH.hubbard_hamiltonian(N, OH.square_lattice(Nx, Ny, True), 1.0, U)
+    Eed = OH.ground_energy(Hed, OH.electron_sector(N, N, 0))
+    assert abs(E - Eed) < 1e-9, (E, Eed)
+
+
+def test_autompo_device_mpo_matches_dense(H):
+    """to_mpo's QN site tensors (block-sparse on the device) expand to the
+    builder's dense tensors: boundary row 'start', column 'done'."""
+    s = H.electron_sites(4)
+    a = H.AutoMPO(s)
+    for i, j in [(1, 2), (2, 3), (3, 4), (1, 4)]:
+        a.add(-1.0, "Cdagup", i, "Cup", j)
+        a.add(-1.0, "Cdagup", j, "Cup", i)
+        a.add(-1.0, "Cdagdn", i, "Cdn", j)
+        a.add(-1.0, "Cdagdn", j, "Cdn", i)
+    for i in range(1, 5):
+        a.add(4.0, "Nupdn", i)
+    W, dims, links = a.dense()
+    mpo = a.to_mpo()
+    for n, (w, t) in enumerate(zip(W, mpo)):
+        got = H.remove_qns(t).store()
+        ref = w[:, :, 1, :] if n == 0 else (w[:, :, :, 0] if n == len(W) - 1 else w)
+        assert np.array_equal(np.asarray(got).reshape(ref.shape, order="F"), ref), n
+
+
+@pytest.mark.parametrize("fromleft", [True, False])
+def test_delta_rho_matches_numpy(H, fromleft):
+    """LocalOp.deltaRho (localop.h:375-406) on dense tensors: drho is the
+    symmetrised X X^T of X = combine * noPrime(AA*L*W1) (or AA*R*W2), so
+    its spectrum and trace equal those of the numpy product."""
+    rng = np.random.default_rng(55 if fromleft else 56)
+    dl, dr, d, k = 6, 5, 2, 3
+    ids = iter(range(700001, 700100))
+
+    def ix(n, tag, plev=0, iid=None):
+        return H.Index(iid or next(ids), n, 1, plev, tag)
+
+    l, s1, s2, r = ix(dl, "l"), ix(d, "s1"), ix(d, "s2"), ix(dr, "r")
+    h0, h1, h2 = ix(k, "h0"), ix(k, "h1"), ix(k, "h2")
+    AAa = rng.standard_normal((dl, d, d, dr))
+    La, Ra = rng.standard_normal((dl, k, dl)), rng.standard_normal((dr, k, dr))
+    W1a, W2a = rng.standard_normal((d, d, k, k)), rng.standard_normal((d, d, k, k))
+
+    def mk(inds, a):
+        return H.dense(inds, np.asarray(a).ravel(order="F"))
+
+    AA = mk([l, s1, s2, r], AAa)
+    L, R = mk([l, h0, l.prime()], La), mk([r, h2, r.prime()], Ra)
+    W1, W2 = mk([s1, s1.prime(), h0, h1], W1a), mk([s2, s2.prime(), h1, h2], W2a)
+    op = H.LocalOp(W1, W2, L, R, False)
+    if fromleft:
+        X = np.einsum("lsur,lhm,stha->mtura", AAa, La, W1a).reshape(dl * d, -1, order="F")
+        C, _ = H.combiner([l, s1])
+    else:
+        X = np.einsum("lsur,rhm,uvah->vmlsa", AAa, Ra, W2a).reshape(d * dr, -1, order="F")
+        C, _ = H.combiner([s2, r])
+    drho, tr = op.delta_rho(AA, C, fromleft)
+    n = X.shape[0]
+    D = np.asarray(drho.store()).reshape(n, n, order="F")
+    ref = X @ X.T
+    assert np.max(np.abs(D - D.T)) < 1e-12
+    assert np.max(np.abs(np.sort(np.linalg.eigvalsh(D)) - np.sort(np.linalg.eigvalsh(ref)))) < 1e-10 * np.max(ref)
+    assert abs(tr - np.trace(ref)) < 1e-10 * np.trace(ref)
