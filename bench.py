@@ -318,6 +318,35 @@ def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0):
             "hpsi_products_last_sweep": sum(r["nprod"] for r in last)}
 
 
+HUBBARD_MAXDIM = [50, 100, 200, 400]
+
+
+def hubbard_leg(Nx: int, Ny: int, maxdims, prec: int, final_fp64: int = 0):
+    """(Nf, Sz)-conserving Hubbard DMRG on the Nx x Ny YPeriodic cylinder
+    (t = 1, U = 4, half filling, cutoff 1e-9, niter 2, noise 1e-6 / 1e-7 on
+    the first two sweeps) in one H*psi form: wall time per sweep and the
+    phase split of the last sweep."""
+    import torch
+
+    from itensor_amd import host as H
+
+    mpo, A = hubbard_cylinder(H, Nx, Ny, 1.0, 4.0, True)
+    noise = [1e-6, 1e-7] + [0.0] * max(0, len(maxdims) - 2)
+    sweeps = [{"MaxDim": int(m), "Cutoff": 1e-9, "Niter": 2, "Noise": z} for m, z in zip(maxdims, noise)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "SyncTiming": True,
+                                               "FinalFP64Sweeps": final_fp64})
+    wall = time.perf_counter() - t0
+    last = [r for r in recs if r["sweep"] == len(maxdims)]
+    return {"energy": E, "wall_s": round(wall, 3), "sec_per_sweep": [round(x, 4) for x in secs],
+            "max_dim": [max(r["dim"] for r in recs if r["sweep"] == k + 1) for k in range(len(maxdims))],
+            "truncerr_max_last_sweep": max(r["truncerr"] for r in last),
+            "last_sweep_split_s": {"env": round(sum(r["t_env"] for r in last), 4),
+                                   "davidson": round(sum(r["t_davidson"] for r in last), 4),
+                                   "svd": round(sum(r["t_svd"] for r in last), 4)}}
+
+
 def measured_traffic(chi: int, prec: str, stage: str):
     """HBM bytes per launch of a stage as rocprofv3 FETCH_SIZE + WRITE_SIZE
     measured them (copy-calibrated) -- read from the committed profile
@@ -365,6 +394,9 @@ def main():
                     help="QN DMRG leg: N:maxdim1,maxdim2,... ('full' = the 10-sweep ramp to 2048); '' skips it")
     ap.add_argument("--dmrg-forms", default="f64,f32,bf16x3,bf16x3+f64last",
                     help="H*psi forms of the DMRG leg ('+f64last': the last sweep in FP64)")
+    ap.add_argument("--hubbard", default="8x4:" + ",".join(str(m) for m in HUBBARD_MAXDIM),
+                    help="Hubbard cylinder leg: NxxNy:maxdim1,maxdim2,... ('' skips it)")
+    ap.add_argument("--hubbard-forms", default="f64,f32", help="H*psi forms of the Hubbard leg")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
@@ -508,6 +540,24 @@ def main():
                     dm["dE_%s_vs_f64" % form] = dm[form]["energy"] - dm["f64"]["energy"]
                     dm["sweep_speedup_%s_vs_f64" % form] = round(dm["f64"]["wall_s"] / dm[form]["wall_s"], 3)
         line["dmrg"] = dm
+
+    if args.hubbard:
+        geo, md_s = args.hubbard.split(":")
+        nx, ny = (int(x) for x in geo.split("x"))
+        mds = [int(x) for x in md_s.split(",")]
+        hb = {"workload": "Hubbard %dx%d YPeriodic cylinder (squareLattice order), t=1, U=4, half filling, "
+                          "(Nf, Sz) QNs, MPO from the host AutoMPO; maxdim %s, cutoff 1e-9, niter 2, noise 1e-6/1e-7 "
+                          "on the first two sweeps; H*psi in the named form, everything else FP64" % (nx, ny, mds)}
+        for form in args.hubbard_forms.split(","):
+            base, _, last = form.partition("+")
+            hb[form] = hubbard_leg(nx, ny, mds, {"f64": 0, "f32": 1, "bf16x3": 2}[base], 1 if last == "f64last" else 0)
+            print("hubbard leg %s done in %.1f s" % (form, hb[form]["wall_s"]), file=sys.stderr, flush=True)
+        if "f64" in hb:
+            for form in args.hubbard_forms.split(","):
+                if form != "f64":
+                    hb["dE_%s_vs_f64" % form] = hb[form]["energy"] - hb["f64"]["energy"]
+                    hb["sweep_speedup_%s_vs_f64" % form] = round(hb["f64"]["wall_s"] / hb[form]["wall_s"], 3)
+        line["hubbard"] = hb
 
     if not args.no_cpu:
         ref, t_cpu, threads, reps = cpu_baseline(psi, L, W, R)

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <numeric>
 #include <queue>
 #include <sstream>
@@ -719,6 +720,34 @@ int64_t emit_tables(const std::vector<Digit> &d, std::vector<int64_t> &tab) {
   return start;
 }
 
+// emit_tables through a cache: blocks of equal shape and strides (most
+// of a QN task list) share one pair of tables
+struct TableCache {
+  std::map<std::vector<int64_t>, int64_t> at;
+  int64_t emit(const std::vector<Digit> &d, std::vector<int64_t> &tab) {
+    std::vector<int64_t> key;
+    key.reserve(3 * d.size());
+    for (auto &x : d) {
+      key.push_back(x.dim);
+      key.push_back(x.s1);
+      key.push_back(x.s2);
+    }
+    auto it = at.find(key);
+    if (it != at.end()) return it->second;
+    const int64_t s = emit_tables(d, tab);
+    at.emplace(std::move(key), s);
+    return s;
+  }
+};
+// largest offset either table of a digit set holds (strides are positive)
+void digits_max(const std::vector<Digit> &d, int64_t &m1, int64_t &m2) {
+  m1 = m2 = 0;
+  for (auto &x : d) {
+    m1 += (x.dim - 1) * x.s1;
+    m2 += (x.dim - 1) * x.s2;
+  }
+}
+
 struct LabelMap {
   // for A index i: contracted partner in B (or -1) / position in C (or -1)
   std::vector<int> a2b, a2c, b2a, b2c;
@@ -899,8 +928,18 @@ int tile_occupancy(int tile, int prec, bool akc, bool bkc) {
 // onto the least loaded worker (what the persistent loop approximates).
 double lpt_makespan(std::vector<double> costs, int64_t slots) {
   if (costs.empty()) return 0;
+  if (int64_t(costs.size()) <= slots) return *std::max_element(costs.begin(), costs.end());
+  if (int64_t(costs.size()) > 4 * slots) {
+    // many more units than slots: LPT ends within one unit of the load
+    // bound, which is what the comparison between tiles needs
+    double sum = 0, mx = 0;
+    for (double c : costs) {
+      sum += c;
+      mx = std::max(mx, c);
+    }
+    return std::max(mx, sum / double(slots) + 0.5 * mx);
+  }
   std::sort(costs.begin(), costs.end(), std::greater<double>());
-  if (int64_t(costs.size()) <= slots) return costs.front();
   std::priority_queue<double, std::vector<double>, std::greater<double>> q;
   for (int64_t i = 0; i < slots; ++i) q.push(0.0);
   double mk = 0;
@@ -938,7 +977,10 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
 
   std::vector<GemmTask> tasks;
   std::vector<int64_t> tab;
+  TableCache tcache;
   tasks.reserve(ntasks);
+  // operand extents (elements) for the autotuner's scratch buffers
+  int64_t extA = 1, extB = 1, extC = 1;
   // A is read along k if its unit-stride digit is a contracted one (else
   // along m); likewise B.  Decided on the first non-empty task.
   bool akc = false, bkc = false, kc_set = false;
@@ -998,15 +1040,24 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     g.M = int32_t(M);
     g.N = int32_t(N);
     g.K = int32_t(K);
-    int64_t t0 = emit_tables(dm, tab);
+    int64_t t0 = tcache.emit(dm, tab);
     g.tAM = t0;
     g.tCM = t0 + M;
-    t0 = emit_tables(dn, tab);
+    t0 = tcache.emit(dn, tab);
     g.tBN = t0;
     g.tCN = t0 + N;
-    t0 = emit_tables(dk, tab);
+    t0 = tcache.emit(dk, tab);
     g.tAK = t0;
     g.tBK = t0 + K;
+    {
+      int64_t am, cm, bn, cn, ak, bk;
+      digits_max(dm, am, cm);
+      digits_max(dn, bn, cn);
+      digits_max(dk, ak, bk);
+      extA = std::max(extA, g.a_off + am + ak + 1);
+      extB = std::max(extB, g.b_off + bk + bn + 1);
+      extC = std::max(extC, g.c_off + cm + cn + 1);
+    }
     flops += 2.0 * double(M) * double(N) * double(K);
     tasks.push_back(g);
   }
@@ -1035,19 +1086,6 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     for (size_t q = i; q < e; ++q) ks += st[q].K;
     groups.push_back({int(i), int(e), st[i].M, st[i].N, ks});
     i = e;
-  }
-
-  // operand extents (elements) for the autotuner's scratch buffers
-  int64_t extA = 1, extB = 1, extC = 1;
-  for (auto &t : st) {
-    auto mx = [&](int64_t start, int64_t n) {
-      int64_t v = 0;
-      for (int64_t i = 0; i < n; ++i) v = std::max(v, tab[size_t(start + i)]);
-      return v;
-    };
-    extA = std::max(extA, t.a_off + mx(t.tAM, t.M) + mx(t.tAK, t.K) + 1);
-    extB = std::max(extB, t.b_off + mx(t.tBK, t.K) + mx(t.tBN, t.N) + 1);
-    extC = std::max(extC, t.c_off + mx(t.tCM, t.M) + mx(t.tCN, t.N) + 1);
   }
 
   const int cus = std::max(1, ctx->num_cus);

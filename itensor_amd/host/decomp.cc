@@ -91,6 +91,7 @@ std::tuple<Real, Real, Real, int> truncate(std::vector<Real> &P, long maxdim, lo
 
 // diag_hermitian (hermitian.cc:40-440), real symmetric input
 Spectrum diag_hermitian(ITensor H, ITensor &U, ITensor &D, Args const &args_in) {
+  HostTimer ht(HT_DIAG_HERMITIAN);
   Args args = args_in;
   if (H.order() != 2) throw ITError("diag_hermitian requires an order 2 tensor");
   Index i1 = H.inds().front(), i2 = H.inds().back();
@@ -501,6 +502,7 @@ std::tuple<ITensor, ITensor> factor(ITensor const &T, IndexSet const &Ais, Args 
 // denmatDecomp (decomp.h:316-418)
 Spectrum denmatDecomp(ITensor const &AA, ITensor &A, ITensor &B, Direction dir, Args args,
                       DeltaRhoFn const &deltaRho) {
+  HostTimer ht(HT_DENMAT);
   const double noise = args.getReal("Noise", 0.0);
   if (noise > 0 && !deltaRho) throw ITError("denmatDecomp: non-zero noise needs the local Hamiltonian");
   if (!args.defined("Tags")) args.add("Tags", std::string("Link"));

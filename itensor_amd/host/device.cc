@@ -1,5 +1,8 @@
 // Process-wide libitx context of the host layer, device buffers and the
 // contraction-precision setting.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 
 #include "itensor.h"
@@ -12,6 +15,35 @@ itx_ctx *g_ctx = nullptr;
 int g_device = 0;
 int g_prec = ITX_PREC_F64;
 }  // namespace
+
+namespace {
+struct TimerTable {
+  double secs[HT_COUNT] = {};
+  long calls[HT_COUNT] = {};
+  bool on = std::getenv("ITX_HOST_TIMERS") != nullptr;
+  ~TimerTable() {
+    if (!on) return;
+    static const char *names[HT_COUNT] = {"qnTaskList",    "itx_plan_qn",     "itx_plan_dense", "contract QN (eager)",
+                                          "contract dense (eager)", "H*psi program build", "LocalOp::product",
+                                          "norm / dot (with wait)", "diag_hermitian", "denmatDecomp",
+                                          "LocalMPO::position"};
+    std::fprintf(stderr, "[itx host timers]\n");
+    for (int i = 0; i < HT_COUNT; ++i)
+      if (calls[i]) std::fprintf(stderr, "  %-26s %9ld calls %10.3f ms\n", names[i], calls[i], secs[i] * 1e3);
+  }
+};
+TimerTable g_timers;
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+HostTimer::HostTimer(HostTimerId id) : id_(id), on_(g_timers.on), t0_(on_ ? now_s() : 0.0) {}
+HostTimer::~HostTimer() {
+  if (!on_) return;
+  g_timers.secs[id_] += now_s() - t0_;
+  ++g_timers.calls[id_];
+}
 
 void check(int rc, const char *what) {
   if (rc != ITX_OK) {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <numeric>
 
+#include <map>
 #include "itensor.h"
 
 namespace itensor {
@@ -113,6 +114,7 @@ void checkArrows(IndexSet const &a, IndexSet const &b, bool shouldMatch = false)
 }
 
 ITensor contractDense(ITensor const &A, ITensor const &B) {
+  HostTimer ht(HT_CONTRACT_DENSE);
   std::vector<int32_t> la, lb, lc;
   computeLabels(A.inds(), B.inds(), la, lb);
   IndexSet N;
@@ -134,6 +136,7 @@ ITensor contractDense(ITensor const &A, ITensor const &B) {
 // deduplicated, with prefix-sum offsets (:221-239).  Structure only.
 QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet const &Bis,
                       BlockOffsets const &Boffs) {
+  HostTimer ht(HT_QN_TASKLIST);
   QNTaskList tl;
   computeLabels(Ais, Bis, tl.la, tl.lb);
   contractIS(Ais, tl.la, Bis, tl.lb, tl.N, tl.lc);
@@ -153,40 +156,76 @@ QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet c
         AtoB[ia] = ib;
         break;
       }
-  struct Triple {
-    int64_t aoff, boff;
-    Block const *a, *b;
-    Block c;
-  };
-  std::vector<Triple> triples;
-  BlockOffsets csizes;
-  Block cblk(rC, 0);
-  for (auto const &aio : Aoffs) {
-    for (int ia = 0; ia < rA; ++ia)
-      if (AtoC[ia] != -1) cblk[AtoC[ia]] = aio.block[ia];
-    for (auto const &bio : Boffs) {
-      bool ok = true;
-      for (int ia = 0; ia < rA && ok; ++ia)
-        if (AtoB[ia] != -1 && aio.block[ia] != bio.block[AtoB[ia]]) ok = false;
-      if (!ok) continue;
-      for (int ib = 0; ib < rB; ++ib)
-        if (BtoC[ib] != -1) cblk[BtoC[ib]] = bio.block[ib];
-      triples.push_back({aio.offset, bio.offset, &aio.block, &bio.block, cblk});
-      csizes.push_back({cblk, prod(blockDims(N, cblk))});
+  // B blocks ordered by their contracted block numbers (in A's index
+  // order): each A block then finds its partners as one equal range, and
+  // the pairs come out in the order of the reference's double loop (A
+  // blocks outer, B blocks in storage order inner)
+  std::vector<int> cA, cB;
+  for (int ia = 0; ia < rA; ++ia)
+    if (AtoB[ia] != -1) {
+      cA.push_back(ia);
+      cB.push_back(AtoB[ia]);
     }
+  const int nc = int(cA.size());
+  std::vector<int> border(Boffs.size());
+  std::iota(border.begin(), border.end(), 0);
+  auto bkey_less = [&](int x, int y) {
+    Block const &bx = Boffs[size_t(x)].block, &by = Boffs[size_t(y)].block;
+    for (int q = 0; q < nc; ++q)
+      if (bx[cB[q]] != by[cB[q]]) return bx[cB[q]] < by[cB[q]];
+    return x < y;  // storage order within a key
+  };
+  std::sort(border.begin(), border.end(), bkey_less);
+  struct Pair {
+    int a, b;
+  };
+  std::vector<Pair> pairs;
+  for (int x = 0; x < int(Aoffs.size()); ++x) {
+    Block const &ab = Aoffs[size_t(x)].block;
+    auto cmpAB = [&](int y, int) {  // B block y's key < A block's key
+      Block const &by = Boffs[size_t(y)].block;
+      for (int q = 0; q < nc; ++q)
+        if (by[cB[q]] != ab[cA[q]]) return by[cB[q]] < ab[cA[q]];
+      return false;
+    };
+    auto cmpBA = [&](int, int y) {  // A block's key < B block y's key
+      Block const &by = Boffs[size_t(y)].block;
+      for (int q = 0; q < nc; ++q)
+        if (ab[cA[q]] != by[cB[q]]) return ab[cA[q]] < by[cB[q]];
+      return false;
+    };
+    auto lo = std::lower_bound(border.begin(), border.end(), 0, cmpAB);
+    auto hi = std::upper_bound(lo, border.end(), 0, cmpBA);
+    for (auto it = lo; it != hi; ++it) pairs.push_back({x, *it});
   }
-  std::sort(csizes.begin(), csizes.end(), [](BlOf const &x, BlOf const &y) { return blockLess(x.block, y.block); });
-  csizes.erase(std::unique(csizes.begin(), csizes.end(), [](BlOf const &x, BlOf const &y) { return x.block == y.block; }),
-               csizes.end());
-  int64_t cur = 0;
-  for (auto &c : csizes) {
-    int64_t sz = c.offset;
-    c.offset = cur;
-    cur += sz;
+  // C blocks: sorted (reference order), deduplicated, prefix-summed sizes
+  Block cblk(rC, 0);
+  auto cblock_of = [&](Pair const &pr) {
+    Block const &ab = Aoffs[size_t(pr.a)].block, &bb = Boffs[size_t(pr.b)].block;
+    for (int ia = 0; ia < rA; ++ia)
+      if (AtoC[ia] != -1) cblk[AtoC[ia]] = ab[ia];
+    for (int ib = 0; ib < rB; ++ib)
+      if (BtoC[ib] != -1) cblk[BtoC[ib]] = bb[ib];
+  };
+  std::map<Block, int, bool (*)(Block const &, Block const &)> cid(blockLess);
+  for (auto const &pr : pairs) {
+    cblock_of(pr);
+    cid.try_emplace(cblk, 0);
   }
-  tl.coffs = std::move(csizes);
-  tl.csize = cur;
-  const int nt = int(triples.size());
+  {
+    int k = 0;
+    int64_t cur = 0;
+    tl.coffs.reserve(cid.size());
+    for (auto &kv : cid) {
+      kv.second = k++;
+      int64_t sz = 1;
+      for (int c = 0; c < rC; ++c) sz *= N[c].blocksize0(kv.first[c]);
+      tl.coffs.push_back({kv.first, cur});
+      cur += sz;
+    }
+    tl.csize = cur;
+  }
+  const int nt = int(pairs.size());
   tl.nt = nt;
   tl.offs.resize(3 * size_t(nt));
   tl.bda.resize(size_t(nt) * rA);
@@ -194,16 +233,16 @@ QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet c
   tl.cfirst.assign(size_t(nt), 0);
   std::vector<char> seen(tl.coffs.size(), 0);
   for (int t = 0; t < nt; ++t) {
-    tl.offs[3 * t] = triples[t].aoff;
-    tl.offs[3 * t + 1] = triples[t].boff;
-    auto it = std::lower_bound(tl.coffs.begin(), tl.coffs.end(), triples[t].c, BlockCmp());
-    tl.offs[3 * t + 2] = it->offset;
-    const size_t ci = size_t(it - tl.coffs.begin());
-    if (!seen[ci]) tl.cfirst[size_t(t)] = seen[ci] = 1;
-    auto da = blockDims(Ais, *triples[t].a);
-    auto db = blockDims(Bis, *triples[t].b);
-    std::copy(da.begin(), da.end(), tl.bda.begin() + size_t(t) * rA);
-    std::copy(db.begin(), db.end(), tl.bdb.begin() + size_t(t) * rB);
+    Pair const &pr = pairs[size_t(t)];
+    cblock_of(pr);
+    const int ci = cid.find(cblk)->second;
+    tl.offs[3 * size_t(t)] = Aoffs[size_t(pr.a)].offset;
+    tl.offs[3 * size_t(t) + 1] = Boffs[size_t(pr.b)].offset;
+    tl.offs[3 * size_t(t) + 2] = tl.coffs[size_t(ci)].offset;
+    if (!seen[size_t(ci)]) tl.cfirst[size_t(t)] = seen[size_t(ci)] = 1;
+    Block const &ab = Aoffs[size_t(pr.a)].block, &bb = Boffs[size_t(pr.b)].block;
+    for (int i = 0; i < rA; ++i) tl.bda[size_t(t) * rA + i] = Ais[i].blocksize0(ab[i]);
+    for (int i = 0; i < rB; ++i) tl.bdb[size_t(t) * rB + i] = Bis[i].blocksize0(bb[i]);
   }
   return tl;
 }
@@ -213,6 +252,7 @@ namespace {
 // the block loop of doTask(Contract,QDense,QDense) (itdata/qdense.cc:671-743)
 // as one grouped launch
 ITensor contractQN(ITensor const &A, ITensor const &B) {
+  HostTimer ht(HT_CONTRACT_QN);
   QNTaskList tl = qnTaskList(A.inds(), A.store().offsets, B.inds(), B.store().offsets);
   auto st = mkQDense(tl.coffs, size_t(tl.csize), false);
   if (tl.nt > 0) {
@@ -441,6 +481,7 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
     contractIS(A.inds(), la, B.inds(), lb, N_, lc);
     auto dA = dimsOf(A.inds()), dB = dimsOf(B.inds()), dC = dimsOf(N_);
     size_ = size_t(prod(dC));
+    HostTimer ht(HT_PLAN_DENSE);
     check(itx_plan_dense(ctx(), int(dA.size()), dA.data(), la.data(), int(dB.size()), dB.data(), lb.data(),
                          int(dC.size()), dC.data(), lc.data(), prec, &plan_),
           "itx_plan_dense");
@@ -454,6 +495,7 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
   const int rA = A.order(), rB = B.order(), rC = int(N_.size());
   if (tl.nt == 0) return;
   if (!perBlock) {
+    HostTimer ht(HT_PLAN_QN);
     check(itx_plan_qn(ctx(), rA, tl.la.data(), rB, tl.lb.data(), rC, tl.lc.data(), tl.nt, tl.offs.data(),
                       tl.bda.data(), tl.bdb.data(), prec, &plan_),
           "itx_plan_qn");
@@ -756,6 +798,7 @@ ITensor operator-(ITensor L, ITensor const &R) {
 }
 
 Real norm(ITensor const &T) {
+  HostTimer ht(HT_NORM_DOT);
   if (!T) throw ITError("norm of default constructed ITensor");
   if (T.store().size == 0) return 0.0;
   double r = 0;
@@ -764,6 +807,7 @@ Real norm(ITensor const &T) {
 }
 
 Real dotDag(ITensor const &A, ITensor const &B) {
+  HostTimer ht(HT_NORM_DOT);
   if (sameIndexSet(A.inds(), B.inds()) &&
       (!A.isQN() || sameOffsets(A.store().offsets, B.store().offsets)) && A.isQN() == B.isQN()) {
     if (A.store().size == 0) return 0.0;
@@ -972,6 +1016,7 @@ ITensor lincomb(std::vector<ITensor const *> const &V, std::vector<Real> const &
 }
 
 std::vector<Real> dotDagMany(std::vector<ITensor const *> const &V, ITensor const &q) {
+  HostTimer ht(HT_NORM_DOT);
   std::vector<Real> out(V.size(), 0.0);
   bool aligned = V.size() <= 16;
   for (size_t k = 0; k < V.size() && aligned; ++k) aligned = alignedStorage(*V[k], q);

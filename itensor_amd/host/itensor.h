@@ -306,6 +306,34 @@ ITensor qdenseFromDense(IndexSet const &is, std::vector<double> const &colmajor,
 // (getContractedOffsets, itdata/qutil.h:93-242).  Task t contracts the A
 // block at offs[3t] with the B block at offs[3t+1] into the C block at
 // offs[3t+2]; cfirst[t] marks the first task of its C block.
+// Host-side phase timers (wall clock, accumulated per process); active when
+// the environment variable ITX_HOST_TIMERS is set, reported to stderr at
+// exit.  Diagnostics only: they add no synchronisation.
+enum HostTimerId {
+  HT_QN_TASKLIST,
+  HT_PLAN_QN,
+  HT_PLAN_DENSE,
+  HT_CONTRACT_QN,
+  HT_CONTRACT_DENSE,
+  HT_PROGRAM_BUILD,
+  HT_PRODUCT,
+  HT_NORM_DOT,
+  HT_DIAG_HERMITIAN,
+  HT_DENMAT,
+  HT_ENV,
+  HT_COUNT
+};
+class HostTimer {
+ public:
+  explicit HostTimer(HostTimerId id);
+  ~HostTimer();
+
+ private:
+  HostTimerId id_;
+  bool on_;
+  double t0_;
+};
+
 struct QNTaskList {
   IndexSet N;
   std::vector<int32_t> la, lb, lc;

@@ -158,6 +158,7 @@ LocalOp::Program *LocalOp::program(ITensor const &phi) const {
   auto ops = chainOps();
   for (auto *o : ops)
     if (!o || !*o || o->isCombiner() || o->order() == 0 || o->isQN() != phi.isQN()) return nullptr;
+  HostTimer ht(HT_PROGRAM_BUILD);
   auto prg = std::make_shared<Program>();
   prg->phiInds = phi.inds();
   prg->qn = phi.isQN();
@@ -194,6 +195,7 @@ LocalOp::Program *LocalOp::program(ITensor const &phi) const {
 }
 
 void LocalOp::product(ITensor const &phi, ITensor &phip) const {
+  HostTimer ht(HT_PRODUCT);
   PrecisionScope ps(prec_);
   if (Program *prg = program(phi)) {
     const size_t n = prg->stages.size();
@@ -259,6 +261,7 @@ void LocalMPO::fuseOps(IndexSet const &phiInds) {
 }
 
 void LocalMPO::position(int b, MPS const &psi) {
+  HostTimer ht(HT_ENV);
   auto t0 = std::chrono::steady_clock::now();
   auto const &H = *H_;
   while (LHlim_ < b - 1) {

@@ -20,6 +20,7 @@ single-particle spectrum of the hopping matrix.
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 
 
 def square_lattice(Nx: int, Ny: int, yperiodic: bool = False):
@@ -42,39 +43,36 @@ def square_lattice(Nx: int, Ny: int, yperiodic: bool = False):
 
 
 def _mode_ops(nmodes: int):
-    """Annihilators c_m (dense, 2^nmodes), mode 0 the least significant bit,
-    with Jordan-Wigner strings over the lower modes."""
-    I2 = np.eye(2)
-    Z = np.diag([1.0, -1.0])
-    a = np.array([[0.0, 1.0], [0.0, 0.0]])  # |0><1|
+    """Annihilators c_m (sparse CSR, 2^nmodes), mode 0 the least significant
+    bit, with Jordan-Wigner strings over the lower modes."""
+    I2 = sp.identity(2, format="csr")
+    Z = sp.diags([1.0, -1.0], format="csr")
+    a = sp.csr_matrix(np.array([[0.0, 1.0], [0.0, 0.0]]))  # |0><1|
     ops = []
     for m in range(nmodes):
-        # kron with the most significant mode first
-        mats = []
-        for k in range(nmodes - 1, -1, -1):
-            mats.append(a if k == m else (Z if k < m else I2))
-        M = mats[0]
-        for x in mats[1:]:
-            M = np.kron(M, x)
+        M = None
+        for k in range(nmodes - 1, -1, -1):  # most significant mode first
+            f = a if k == m else (Z if k < m else I2)
+            M = f if M is None else sp.kron(M, f, format="csr")
         ops.append(M)
     return ops
 
 
-def hubbard_hamiltonian(N: int, bonds, t: float, U: float) -> np.ndarray:
+def hubbard_hamiltonian(N: int, bonds, t: float, U: float):
     """H = -t sum_<ij>,s (c+_is c_js + h.c.) + U sum_i n_i,up n_i,dn on N
-    sites (4^N x 4^N, site basis as in the module docstring)."""
+    sites (sparse 4^N x 4^N, site basis as in the module docstring)."""
     c = _mode_ops(2 * N)
-    H = np.zeros((4 ** N, 4 ** N))
+    H = sp.csr_matrix((4 ** N, 4 ** N))
     for (i, j) in bonds:
         for s in (0, 1):
             ci, cj = c[2 * (i - 1) + s], c[2 * (j - 1) + s]
             hop = ci.T @ cj
-            H -= t * (hop + hop.T)
+            H = H - t * (hop + hop.T)
     for i in range(1, N + 1):
         nu = c[2 * (i - 1)].T @ c[2 * (i - 1)]
         nd = c[2 * (i - 1) + 1].T @ c[2 * (i - 1) + 1]
-        H += U * (nu @ nd)
-    return H
+        H = H + U * (nu @ nd)
+    return H.tocsr()
 
 
 def electron_sector(N: int, nf: int, sz2: int) -> np.ndarray:
@@ -89,9 +87,11 @@ def electron_sector(N: int, nf: int, sz2: int) -> np.ndarray:
     return np.nonzero((nu + nd == nf) & (nu - nd == sz2))[0]
 
 
-def ground_energy(H: np.ndarray, sector=None) -> float:
+def ground_energy(H, sector=None) -> float:
+    """Lowest eigenvalue (dense solver on the sector block)."""
     if sector is not None:
-        H = H[np.ix_(sector, sector)]
+        H = H[sector][:, sector]
+    H = H.toarray() if sp.issparse(H) else H
     return float(np.linalg.eigvalsh(H)[0])
 
 

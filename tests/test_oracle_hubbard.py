@@ -24,7 +24,7 @@ def test_square_lattice_bond_counts():
 @pytest.mark.parametrize("U", [0.0, 1.0, 4.0, 10.0])
 def test_hubbard_dimer_closed_form(U):
     t = 1.0
-    H = OH.hubbard_hamiltonian(2, [(1, 2)], t, U)
+    H = OH.hubbard_hamiltonian(2, [(1, 2)], t, U).toarray()
     assert np.allclose(H, H.T)
     e = OH.ground_energy(H, OH.electron_sector(2, 2, 0))
     assert abs(e - (U - math.sqrt(U * U + 16 * t * t)) / 2) < 1e-12
@@ -108,7 +108,7 @@ def test_autompo_hubbard_matches_ed(bonds):
         a.add(U, "Nupdn", i)
     W, dims, links = a.dense()
     M = mpo_matrix(W)
-    ref = OH.hubbard_hamiltonian(N, bonds, t, U)
+    ref = OH.hubbard_hamiltonian(N, bonds, t, U).toarray()
     assert np.max(np.abs(M - ref)) < 1e-13
     check_flux(W, links, ELEC_Q)
 
@@ -128,9 +128,10 @@ def test_autompo_mixed_terms_match_ed():
     c = OH._mode_ops(2 * N)
     n = [x.T @ x for x in c]
     ref = 0.7 * (c[7].T @ c[1] + c[1].T @ c[7])
-    ref += -0.3 * (n[0] + n[1]) @ (n[4] + n[5])
-    ref += 1.5 * n[2]
-    ref += 0.2 * (0.5 * (n[2] - n[3])) @ (0.5 * (n[4] - n[5]))
+    ref = ref - 0.3 * (n[0] + n[1]) @ (n[4] + n[5])
+    ref = ref + 1.5 * n[2]
+    ref = ref + 0.2 * (0.5 * (n[2] - n[3])) @ (0.5 * (n[4] - n[5]))
+    ref = ref.toarray()
     assert np.max(np.abs(M - ref)) < 1e-13
 
 
