@@ -672,11 +672,24 @@ __global__ __launch_bounds__(SM_T) void k_reduce_tiles(const double *__restrict_
 struct Digit {
   int64_t dim, s1, s2;
 };
+// A digit set in place (at most ITX_MAX_RANK digits): planning handles one
+// per operand side and task, so it must not allocate
+struct Digits {
+  Digit d[ITX_MAX_RANK];
+  int n = 0;
+  void push_back(Digit x) { d[n++] = x; }
+  bool empty() const { return n == 0; }
+  int size() const { return n; }
+  const Digit &operator[](int i) const { return d[i]; }
+  Digit &back() { return d[n - 1]; }
+  const Digit *begin() const { return d; }
+  const Digit *end() const { return d + n; }
+};
 
 // Remove unit digits and fuse neighbours that are contiguous in both
 // operands (the same simplification CProps makes before its GEMM).
-void simplify(std::vector<Digit> &d) {
-  std::vector<Digit> o;
+void simplify(Digits &d) {
+  Digits o;
   for (auto &x : d) {
     if (x.dim == 1) continue;
     if (!o.empty() && o.back().s1 * o.back().dim == x.s1 && o.back().s2 * o.back().dim == x.s2) {
@@ -685,10 +698,10 @@ void simplify(std::vector<Digit> &d) {
       o.push_back(x);
     }
   }
-  d.swap(o);
+  d = o;
 }
 
-int64_t digits_size(const std::vector<Digit> &d) {
+int64_t digits_size(const Digits &d) {
   int64_t s = 1;
   for (auto &x : d) s *= x.dim;
   return s;
@@ -696,13 +709,13 @@ int64_t digits_size(const std::vector<Digit> &d) {
 
 // Append the two offset tables of a digit set to tab; returns the start of
 // the first table (the second starts size() later).
-int64_t emit_tables(const std::vector<Digit> &d, std::vector<int64_t> &tab) {
+int64_t emit_tables(const Digits &d, std::vector<int64_t> &tab) {
   const int64_t n = digits_size(d);
   const int64_t start = int64_t(tab.size());
   tab.resize(tab.size() + 2 * size_t(n));
   int64_t *t1 = tab.data() + start, *t2 = t1 + n;
-  const int r = int(d.size());
-  std::vector<int64_t> ctr(r, 0);
+  const int r = d.size();
+  int64_t ctr[ITX_MAX_RANK] = {};
   int64_t o1 = 0, o2 = 0;
   for (int64_t x = 0; x < n; ++x) {
     t1[x] = o1;
@@ -723,24 +736,31 @@ int64_t emit_tables(const std::vector<Digit> &d, std::vector<int64_t> &tab) {
 // emit_tables through a cache: blocks of equal shape and strides (most
 // of a QN task list) share one pair of tables
 struct TableCache {
-  std::map<std::vector<int64_t>, int64_t> at;
-  int64_t emit(const std::vector<Digit> &d, std::vector<int64_t> &tab) {
-    std::vector<int64_t> key;
-    key.reserve(3 * d.size());
-    for (auto &x : d) {
-      key.push_back(x.dim);
-      key.push_back(x.s1);
-      key.push_back(x.s2);
+  struct Key {
+    Digits d;
+    bool operator<(const Key &o) const {
+      if (d.n != o.d.n) return d.n < o.d.n;
+      for (int i = 0; i < d.n; ++i) {
+        const Digit &x = d.d[i], &y = o.d.d[i];
+        if (x.dim != y.dim) return x.dim < y.dim;
+        if (x.s1 != y.s1) return x.s1 < y.s1;
+        if (x.s2 != y.s2) return x.s2 < y.s2;
+      }
+      return false;
     }
-    auto it = at.find(key);
+  };
+  std::map<Key, int64_t> at;
+  int64_t emit(const Digits &d, std::vector<int64_t> &tab) {
+    const Key k{d};
+    auto it = at.find(k);
     if (it != at.end()) return it->second;
     const int64_t s = emit_tables(d, tab);
-    at.emplace(std::move(key), s);
+    at.emplace(k, s);
     return s;
   }
 };
 // largest offset either table of a digit set holds (strides are positive)
-void digits_max(const std::vector<Digit> &d, int64_t &m1, int64_t &m2) {
+void digits_max(const Digits &d, int64_t &m1, int64_t &m2) {
   m1 = m2 = 0;
   for (auto &x : d) {
     m1 += (x.dim - 1) * x.s1;
@@ -1006,7 +1026,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     colmajor_strides(rA, dA, sA);
     colmajor_strides(rB, dB, sB);
     colmajor_strides(rC, dC, sC);
-    std::vector<Digit> dm, dn, dk;
+    Digits dm, dn, dk;
     for (int i = 0; i < rA; ++i) {
       if (lm.a2c[i] >= 0) dm.push_back({dA[i], sA[i], sC[lm.a2c[i]]});
       else dk.push_back({dA[i], sA[i], sB[lm.a2b[i]]});

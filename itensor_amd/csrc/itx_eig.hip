@@ -163,44 +163,19 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int NMAX>
-__global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tasks, const double *__restrict__ Abase,
-                                                 double *__restrict__ Zbase) {
-  constexpr int LD = NMAX + 1;
-  __shared__ double S[NMAX * LD];
-  __shared__ double d[NMAX], e[NMAX], tau[NMAX], pv[NMAX];
-  __shared__ double part[(ES / 64) * NMAX];
-
-  const EigTask tk = tasks[blockIdx.x];
-  const int n = tk.n;
+// Householder reduction of the symmetric n x n matrix S (element (r, c) at
+// S[c * LD + r], LDS or global memory) to tridiagonal form (d, e), then Q
+// formed in place in S.  ES threads; PL = rows per lane (n <= 64 PL).
+template <int PL>
+__device__ __forceinline__ void reduce_form_q(double *__restrict__ S, int LD, int n, double *d, double *e,
+                                              double *tau, double *pv, double *part) {
+  constexpr int NMAX = 64 * PL;
   const int tid = threadIdx.x;
-  const double *__restrict__ A = Abase + tk.a_off;
-  if (n <= 0 || n > NMAX) return;  // the host sorts matrices into size classes
-  // staged through registers, eight loads in flight per thread
-  for (int x0 = 0; x0 < n * n; x0 += 8 * ES) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int x = x0 + u * ES + tid;
-      v[u] = x < n * n ? A[x] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int x = x0 + u * ES + tid;
-      if (x < n * n) {
-        const int c = x / n, r = x - c * n;
-        S[c * LD + r] = v[u];
-      }
-    }
-  }
-  __syncthreads();
-
   // ---- 1. Householder tridiagonalisation ----
   // Every wave computes the reflector redundantly (lanes over rows, shuffle
   // sums), the product p = tau S v is split over the four waves by column
   // residue mod 4, and the rank-2 update runs waves over columns, lanes
   // over rows: three barriers per step.
-  constexpr int PL = (NMAX + 63) / 64;  // rows per lane
   constexpr int NW = ES / 64;
   const int lane = tid & 63, wv = tid >> 6;
   for (int k = 0; k + 2 < n; ++k) {
@@ -354,6 +329,41 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
     S[i * LD] = (i == 0) ? 1.0 : 0.0;  // row 0
   }
   __syncthreads();
+
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tasks, const double *__restrict__ Abase,
+                                                 double *__restrict__ Zbase) {
+  constexpr int LD = NMAX + 1;
+  __shared__ double S[NMAX * LD];
+  __shared__ double d[NMAX], e[NMAX], tau[NMAX], pv[NMAX];
+  __shared__ double part[(ES / 64) * NMAX];
+
+  const EigTask tk = tasks[blockIdx.x];
+  const int n = tk.n;
+  const int tid = threadIdx.x;
+  const double *__restrict__ A = Abase + tk.a_off;
+  if (n <= 0 || n > NMAX) return;  // the host sorts matrices into size classes
+  // staged through registers, eight loads in flight per thread
+  for (int x0 = 0; x0 < n * n; x0 += 8 * ES) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int x = x0 + u * ES + tid;
+      v[u] = x < n * n ? A[x] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int x = x0 + u * ES + tid;
+      if (x < n * n) {
+        const int c = x / n, r = x - c * n;
+        S[c * LD + r] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  reduce_form_q<NMAX / 64>(S, LD, n, d, e, tau, pv, part);
 
   // ---- 3. Q (column-major) and (d, e) out; the host runs the QL recurrence ----
   double *Z = Zbase + tk.z_off;
@@ -593,12 +603,10 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
     else
       tg.push_back(t);
   }
-  const int ntask = int(t64.size() + t128.size() + tg.size());
-  if (ntask == 0) return ITX_OK;
   std::vector<EigTask> all;
-  all.insert(all.end(), t64.begin(), t64.end());
-  all.insert(all.end(), t128.begin(), t128.end());
-  all.insert(all.end(), tg.begin(), tg.end());
+  for (auto *v : {&t64, &t128, &tg}) all.insert(all.end(), v->begin(), v->end());
+  const int ntask = int(all.size());
+  if (ntask == 0) return ITX_OK;
   for (auto &t : all) t.z_off += desize;
   const int64_t zsize = desize + qsize;
   const size_t tbytes = sizeof(EigTask) * size_t(ntask);
@@ -614,10 +622,15 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   }
   const EigTask *dt = reinterpret_cast<const EigTask *>(wsb);
   double *Z = reinterpret_cast<double *>(wsb + zoff);
-  const int n64 = int(t64.size()), n128 = int(t128.size()), ng = int(tg.size());
-  if (n64) hipLaunchKernelGGL(k_syev_lds<64>, dim3(n64), dim3(ES), 0, ctx->stream, dt, Abase, Z);
-  if (n128) hipLaunchKernelGGL(k_syev_lds<128>, dim3(n128), dim3(ES), 0, ctx->stream, dt + n64, Abase, Z);
-  if (ng) hipLaunchKernelGGL(k_syev, dim3(ng), dim3(ET), 0, ctx->stream, dt + n64 + n128, Abase, Wbase, Z);
+  {
+    const EigTask *q = dt;
+    if (!t64.empty()) hipLaunchKernelGGL(k_syev_lds<64>, dim3(unsigned(t64.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
+    q += t64.size();
+    if (!t128.empty())
+      hipLaunchKernelGGL(k_syev_lds<128>, dim3(unsigned(t128.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
+    q += t128.size();
+    if (!tg.empty()) hipLaunchKernelGGL(k_syev, dim3(unsigned(tg.size())), dim3(ET), 0, ctx->stream, q, Abase, Wbase, Z);
+  }
   hipError_t e = hipGetLastError();
   // the tridiagonals of every matrix to the host (one wait)
   std::vector<double> de(static_cast<size_t>(desize));

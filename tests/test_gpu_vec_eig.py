@@ -55,15 +55,15 @@ def test_permute_add(ctx, dev, dims, perm):
     assert relerr(dev.get(dY, dims), np.transpose(X, perm)) == 0.0
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 10, 64, 65, 100, 128, 129, 256, 257, 400])
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 64, 65, 100, 128, 129, 256, 257, 400, 512, 513])
 def test_syev(ctx, dev, n):
     """Single matrices of every size class (reduction LDS-resident up to 64
-    and up to 128, global above; rotations applied to Q rows staged in LDS
-    up to 256, in global memory above): eigenvalues vs numpy, orthonormal
-    vectors, reconstruction, descending order."""
+    and up to 128, in global memory above; rotations applied to Q rows
+    staged in LDS up to 256, in global memory above): eigenvalues vs numpy,
+    orthonormal vectors, reconstruction, descending order."""
     rng = np.random.default_rng(1000 + n)
     X = rng.standard_normal((n, n))
-    if n in (64, 128, 256):
+    if n in (64, 128, 256, 512):
         # density-matrix-like: PSD with exact zero eigenvalues and degeneracies
         X = X[:, :20] @ X[:, :20].T
         X[:, :2] = 0
@@ -87,7 +87,7 @@ def test_syev_batched_mixed_classes(ctx, dev):
     """One batched call over matrices of all three size classes, including
     degenerate and rank-deficient ones, stored back to back."""
     rng = np.random.default_rng(77)
-    ns = [5, 64, 1, 90, 200, 33, 128, 0, 150, 300, 256, 2]
+    ns = [5, 64, 1, 90, 200, 33, 128, 0, 150, 300, 256, 2, 520]
     mats = []
     for k, n in enumerate(ns):
         X = rng.standard_normal((n, n))
