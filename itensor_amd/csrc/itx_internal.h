@@ -98,8 +98,10 @@ struct itx_ctx {
   // reduction scratch (partials + results)
   double *red = nullptr;
   // pinned host memory: reduction results written by the device
-  // (hres, device view hres_dev) and the staging arena of uploads
+  // (hres, device view hres_dev; coherent, with a completion flag after
+  // the HRES_SLOTS result slots) and the staging arena of uploads
   double *hres = nullptr, *hres_dev = nullptr;
+  uint64_t hres_seq = 0;  // completion flag value of the last host reduction
   char *stage = nullptr;
   size_t stage_cap = 0, stage_off = 0;
   // plan cache for the one-shot contraction entry points

@@ -127,14 +127,20 @@ int itx_create(int device, itx_ctx **out) {
   ctx->device = device;
   ctx->num_cus = prop.multiProcessorCount;
   ctx->stream = nullptr;
+  // partial sums, then 64 doubles' worth of arrival counters (zero between
+  // reductions: the last block of each reduction resets its own)
   e = hipMalloc(&ctx->red, sizeof(double) * (itx::RED_SLOTS + 64));
+  if (e == hipSuccess) e = hipMemset(ctx->red + itx::RED_SLOTS, 0, sizeof(double) * 64);
   if (e != hipSuccess) {
     g_create_err = std::string("hipMalloc(scratch): ") + hipGetErrorString(e);
     delete ctx;
     return ITX_ERR_NOMEM;
   }
   void *hp = nullptr;
-  e = hipHostMalloc(&hp, sizeof(double) * itx::HRES_SLOTS, hipHostMallocDefault);
+  // coherent: device writes become visible to the host without a stream
+  // synchronisation (the completion flag is polled)
+  e = hipHostMalloc(&hp, sizeof(double) * (itx::HRES_SLOTS + 1), hipHostMallocMapped | hipHostMallocCoherent);
+  if (e == hipSuccess) std::memset(hp, 0, sizeof(double) * (itx::HRES_SLOTS + 1));
   void *hd = nullptr;
   if (e == hipSuccess) e = hipHostGetDevicePointer(&hd, hp, 0);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&ctx->stage), kStageBytes, hipHostMallocDefault);

@@ -8,6 +8,8 @@
 // and the Davidson dots eltC(dag(V)*q) (iterativesolvers.h:169,315,393).
 #include <cstring>
 
+#include <atomic>
+#include <chrono>
 #include "itx_internal.h"
 
 namespace itx {
@@ -63,6 +65,46 @@ __global__ __launch_bounds__(VT) void k_dot_final(const double *__restrict__ par
   for (int b = threadIdx.x; b < nb; b += VT) s += partial[size_t(j) * nb + b];
   s = block_sum(s);
   if (threadIdx.x == 0) out[j] = do_sqrt ? sqrt(s) : s;
+}
+
+// Single-launch form for results read by the host: blocks write partials,
+// the last block to arrive for vector j (arrival counter, fence) sums them
+// in the same fixed order as k_dot_final and resets the counter; the last
+// vector to finish sets the host-visible completion flag to seq after a
+// system-scope fence, so the host polls one word instead of synchronising
+// the stream.
+__global__ __launch_bounds__(VT) void k_dot_host(VPtrs V, const double *__restrict__ y, size_t n,
+                                                 double *partial, unsigned *cnt, double *out, int do_sqrt,
+                                                 unsigned long long *flag, unsigned long long seq) {
+  __shared__ int last;
+  const int j = blockIdx.y, nb = gridDim.x, nv = gridDim.y;
+  const double *__restrict__ x = V.p[j];
+  double s = 0;
+  const size_t stride = size_t(nb) * VT;
+  for (size_t i = size_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) s += x[i] * y[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    partial[size_t(j) * nb + blockIdx.x] = s;
+    __threadfence();
+    last = (atomicAdd(&cnt[j], 1u) == unsigned(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double *pv = partial + size_t(j) * nb;
+  double t = 0;
+  for (int b = threadIdx.x; b < nb; b += VT) t += pv[b];
+  t = block_sum(t);
+  if (threadIdx.x == 0) {
+    out[j] = do_sqrt ? sqrt(t) : t;
+    cnt[j] = 0u;
+    __threadfence_system();
+    if (atomicAdd(&cnt[MAXV], 1u) == unsigned(nv - 1)) {
+      cnt[MAXV] = 0u;
+      __threadfence_system();
+      *reinterpret_cast<volatile unsigned long long *>(flag) = seq;
+    }
+  }
 }
 
 __global__ __launch_bounds__(VT) void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y,
@@ -160,11 +202,36 @@ int multi_dot_dev(itx_ctx *ctx, const double *const *V, int nv, const double *y,
   return ITX_OK;
 }
 
-// reduction results land in the context's pinned result slots; one
-// stream synchronisation makes them visible to the host
-int to_host(itx_ctx *ctx, double *hdst, int count) {
-  ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (int j = 0; j < count; ++j) hdst[j] = ctx->hres[j];
+// V_j . y (or |y| with do_sqrt) for j < nv into the host: one launch into
+// the coherent result slots, then a poll of the completion flag.  A poll
+// that runs past 50 ms falls back to synchronising the stream, which also
+// reports a failed launch.
+int dots_to_host(itx_ctx *ctx, const double *const *V, int nv, const double *y, size_t n, double *hdst,
+                 int do_sqrt) {
+  if (nv <= 0) return ITX_OK;
+  if (nv > MAXV) return set_err(ctx, ITX_ERR_UNSUPPORTED, "multi_dot: more than 16 vectors");
+  VPtrs vp{};
+  for (int j = 0; j < nv; ++j) vp.p[j] = V[j];
+  int nbx = int(grid_for(n));
+  const int cap = RED_SLOTS / nv;
+  if (nbx > cap) nbx = cap;
+  const unsigned long long seq = ++ctx->hres_seq;
+  volatile unsigned long long *hflag = reinterpret_cast<volatile unsigned long long *>(ctx->hres + HRES_SLOTS);
+  auto *dflag = reinterpret_cast<unsigned long long *>(ctx->hres_dev + HRES_SLOTS);
+  auto *cnt = reinterpret_cast<unsigned *>(ctx->red + RED_SLOTS);
+  hipLaunchKernelGGL(k_dot_host, dim3(nbx, nv), dim3(VT), 0, ctx->stream, vp, y, n, ctx->red, cnt, ctx->hres_dev,
+                     do_sqrt, dflag, seq);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  for (long it = 0; *hflag != seq; ++it) {
+    if ((it & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (*hflag != seq) return set_err(ctx, ITX_ERR_HIP, "reduction: completion flag not set");
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  for (int j = 0; j < nv; ++j) hdst[j] = ctx->hres[j];
   return ITX_OK;
 }
 
@@ -181,25 +248,20 @@ int itx_dot_dev(itx_ctx *ctx, const double *x, const double *y, size_t n, double
 
 int itx_dot(itx_ctx *ctx, const double *x, const double *y, size_t n, double *result) {
   if (!ctx || !result) return ITX_ERR_ARG;
-  int rc = itx_dot_dev(ctx, x, y, n, ctx->hres_dev);
-  if (rc) return rc;
-  return itx::to_host(ctx, result, 1);
+  const double *V[1] = {x};
+  return itx::dots_to_host(ctx, V, 1, y, n, result, 0);
 }
 
 int itx_nrm2(itx_ctx *ctx, const double *x, size_t n, double *result) {
   if (!ctx || !result) return ITX_ERR_ARG;
   const double *V[1] = {x};
-  int rc = itx::multi_dot_dev(ctx, V, 1, x, n, ctx->hres_dev, 1);
-  if (rc) return rc;
-  return itx::to_host(ctx, result, 1);
+  return itx::dots_to_host(ctx, V, 1, x, n, result, 1);
 }
 
 int itx_multi_dot(itx_ctx *ctx, const double *const *V, int nv, const double *y, size_t n, double *out) {
   if (!ctx || !out || nv < 0 || nv > 16) return ITX_ERR_ARG;
   if (nv == 0) return ITX_OK;
-  int rc = itx::multi_dot_dev(ctx, V, nv, y, n, ctx->hres_dev, 0);
-  if (rc) return rc;
-  return itx::to_host(ctx, out, nv);
+  return itx::dots_to_host(ctx, V, nv, y, n, out, 0);
 }
 
 int itx_axpy(itx_ctx *ctx, double a, const double *x, double *y, size_t n) {
