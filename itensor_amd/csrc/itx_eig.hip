@@ -9,9 +9,10 @@
 // implicit QL iteration with Wilkinson-type shift on T (Bowdler, Martin,
 // Reinsch & Wilkinson, Numer. Math. 11 (1968) 293, "tql2"), whose plane
 // rotations are applied to Q.  The work is split the way hybrid symmetric
-// eigensolvers split it: the O(n^3) parts on the device (reduction and Q in
-// one launch, one workgroup per matrix, matrices up to 128 resident in
-// LDS; the rotations applied to Q's rows in parallel in a second launch),
+// eigensolvers split it: the O(n^3) parts on the device (reduction and Q:
+// one workgroup per matrix with the matrix resident in LDS up to 128, one
+// cooperative launch over the whole device per larger matrix; the
+// rotations applied to Q's rows in parallel in a second launch),
 // the O(n^2) scalar QL recurrence -- a chain of dependent flops that one
 // host core runs far faster than one GPU lane -- on the host in between.
 #include <cfloat>
@@ -20,13 +21,15 @@
 #include <algorithm>
 #include <cstring>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "itx_internal.h"
 
 namespace itx {
 namespace {
 
-constexpr int ET = 512;       // threads per matrix (reduction above 128)
-constexpr int EMAX = 2048;    // largest n (d, e held in LDS)
+constexpr int ET = 512;       // threads per matrix of the batched global-memory reduction
+constexpr int EMAX = 2048;    // largest n
 constexpr int QL_MAXIT = 60;  // QL sweeps per eigenvalue before giving up
 
 struct EigTask {
@@ -50,6 +53,9 @@ __device__ __forceinline__ double bsum(double v, double *red) {
 
 // Reduction and Q for n > 128: A in global memory (overwritten by the
 // reflectors), W as scratch for p, Q into the workspace.
+// One workgroup per matrix over global memory (n > 128): used when a batch
+// holds several such matrices that are cheaper side by side than one after
+// the other through the cooperative kernel.
 __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
                                              double *__restrict__ Wbase, double *__restrict__ Zbase) {
   __shared__ double d[EMAX], e[EMAX], tau_[EMAX];
@@ -147,6 +153,193 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
   for (int i = tid; i < n; i += ET) {
     DE[i] = d[i];
     DE[n + i] = e[i];
+  }
+}
+
+// ---- multi-workgroup reduction for n > 128 (cooperative launch) ---------
+// The same Householder reduction and backward accumulation of Q, spread
+// over the whole device: the trailing block is cut into 64 x 64 tiles that
+// the workgroups walk (persistent loop), and grid-wide barriers separate
+// the phases of a step.  Householder step: every workgroup forms the
+// reflector itself (same data, same reduction order, hence the same branch
+// everywhere); tiles write partial products of p = tau S v per column tile
+// [barrier]; row tiles sum them into p and partial dots v.p [barrier];
+// every workgroup forms K = tau/2 v.p and updates its tiles
+// S -= v w^T + w v^T, w = p - K v [barrier].  Q step: partial column dots
+// v^T Z per row tile [barrier]; each tile sums the 64 column dots it needs
+// and applies the rank-1 update [barrier].  Every workgroup passes every
+// barrier.
+constexpr int CT = 256;  // threads per workgroup (4 waves)
+constexpr int CB = 64;   // tile edge
+
+__device__ __forceinline__ double csum(double v, double *red) {  // block sum, CT threads
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0;
+  for (int i = 0; i < CT / 64; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+// workspace doubles of k_syev_coop for order n
+inline size_t coop_ws(int n) {
+  const size_t nt = size_t((n + CB - 1) / CB);
+  return nt * size_t(n) + 3 * size_t(n) + nt;
+}
+
+// A (n x n, column-major; the reflectors are left in it), Z: Q column-major
+// (leading dimension n), DE: (d, e), wk: coop_ws(n) doubles
+__global__ __launch_bounds__(CT) void k_syev_coop(double *__restrict__ A, int n, double *__restrict__ Z,
+                                                  double *__restrict__ DE, double *__restrict__ wk) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double red[CT / 64];
+  __shared__ double sh[CT];
+  __shared__ double scol[CB];
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nt = (n + CB - 1) / CB;
+  double *part = wk;                     // nt x n partial sums
+  double *pvec = wk + size_t(nt) * n;    // n
+  double *tauv = pvec + n;               // n
+  double *ev = tauv + n;                 // n
+  double *dotp = ev + n;                 // nt
+
+  // ---- 1. Householder tridiagonalisation ----
+  for (int k = 0; k + 2 < n; ++k) {
+    const int L = n - k - 1, o = k + 1;
+    const int ntl = (L + CB - 1) / CB;
+    double *col = A + size_t(k) * n + o;  // x = A(k+1:, k)
+    double sg = 0;
+    for (int i = 1 + tid; i < L; i += CT) sg += col[i] * col[i];
+    const double sigma = csum(sg, red);
+    const double x0 = col[0];
+    double t = 0, alpha = x0, v0 = x0;
+    if (sigma > 0) {
+      const double mu = sqrt(x0 * x0 + sigma);
+      v0 = (x0 <= 0) ? x0 - mu : x0 + mu;
+      alpha = (x0 <= 0) ? mu : -mu;
+      t = 2.0 / (v0 * v0 + sigma);
+    }
+    if (t == 0) {  // the same on every workgroup
+      if (g == 0 && tid == 0) {
+        ev[k] = alpha;
+        tauv[k] = 0;
+      }
+      continue;
+    }
+    // partial products over column tiles: part[tj * n + i]
+    for (int tt = g; tt < ntl * ntl; tt += G) {
+      const int ti = tt % ntl, tj = tt / ntl;
+      const int i = ti * CB + lane;
+      double acc = 0;
+      if (i < L)
+        for (int jj = wv; jj < CB; jj += CT / 64) {
+          const int j = tj * CB + jj;
+          if (j < L) acc += A[size_t(o + j) * n + o + i] * (j == 0 ? v0 : col[j]);
+        }
+      sh[tid] = acc;
+      __syncthreads();
+      if (wv == 0 && i < L) part[size_t(tj) * n + i] = (sh[lane] + sh[64 + lane]) + (sh[128 + lane] + sh[192 + lane]);
+      __syncthreads();
+    }
+    grid.sync();
+    // p on row tiles, with the partial dots v.p
+    for (int ti = g; ti < ntl; ti += G) {
+      double dv = 0;
+      if (tid < CB) {
+        const int i = ti * CB + tid;
+        if (i < L) {
+          double pi = 0;
+          for (int q = 0; q < ntl; ++q) pi += part[size_t(q) * n + i];
+          pi *= t;
+          pvec[i] = pi;
+          dv = (i == 0 ? v0 : col[i]) * pi;
+        }
+      }
+      dv = csum(dv, red);
+      if (tid == 0) dotp[ti] = dv;
+    }
+    grid.sync();
+    double ks = 0;
+    for (int q = tid; q < ntl; q += CT) ks += dotp[q];
+    const double Kc = 0.5 * t * csum(ks, red);
+    // S -= v w^T + w v^T on this workgroup's tiles
+    for (int tt = g; tt < ntl * ntl; tt += G) {
+      const int ti = tt % ntl, tj = tt / ntl;
+      const int i = ti * CB + lane;
+      if (i >= L) continue;
+      const double vi = (i == 0 ? v0 : col[i]), wi = pvec[i] - Kc * vi;
+      for (int jj = wv; jj < CB; jj += CT / 64) {
+        const int j = tj * CB + jj;
+        if (j >= L) break;
+        const double vj = (j == 0 ? v0 : col[j]), wj = pvec[j] - Kc * vj;
+        A[size_t(o + j) * n + o + i] -= vi * wj + wi * vj;
+      }
+    }
+    if (g == 0 && tid == 0) {
+      ev[k] = alpha;
+      tauv[k] = t;
+    }
+    grid.sync();  // every workgroup is done with col[0] before it becomes v0
+    if (g == 0 && tid == 0) col[0] = v0;
+  }
+  grid.sync();
+  for (int i = g * CT + tid; i < n; i += G * CT) {
+    DE[i] = A[size_t(i) * n + i];
+    DE[n + i] = (i + 2 < n) ? ev[i] : (i == n - 2 ? A[size_t(n - 2) * n + (n - 1)] : 0.0);
+  }
+
+  // ---- 2. Q = H_0 ... H_{n-3}, accumulated backwards in Z ----
+  for (size_t x = size_t(g) * CT + tid; x < size_t(n) * n; x += size_t(G) * CT)
+    Z[x] = ((x / n) == (x % n)) ? 1.0 : 0.0;
+  grid.sync();
+  for (int k = n - 3; k >= 0; --k) {
+    const double t = tauv[k];
+    if (t == 0) continue;  // the same on every workgroup
+    const int j0 = k + 1, L = n - j0;
+    const int ntl = (L + CB - 1) / CB;
+    const double *v = A + size_t(k) * n + j0;
+    // part[ti * n + c] = sum over row tile ti of v_i Z(j0+i, j0+c)
+    for (int tt = g; tt < ntl * ntl; tt += G) {
+      const int ti = tt % ntl, tj = tt / ntl;
+      const int i = ti * CB + lane;
+      const double vi = i < L ? v[i] : 0.0;
+      for (int cc = wv; cc < CB; cc += CT / 64) {
+        const int c = tj * CB + cc;
+        if (c >= L) break;
+        double a = i < L ? vi * Z[size_t(j0 + c) * n + j0 + i] : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) part[size_t(ti) * n + c] = a;
+      }
+    }
+    grid.sync();
+    for (int tt = g; tt < ntl * ntl; tt += G) {
+      const int ti = tt % ntl, tj = tt / ntl;
+      if (tid < CB) {  // the 64 column dots of this tile
+        const int c = tj * CB + tid;
+        double sc = 0;
+        if (c < L)
+          for (int q = 0; q < ntl; ++q) sc += part[size_t(q) * n + c];
+        scol[tid] = t * sc;
+      }
+      __syncthreads();
+      const int i = ti * CB + lane;
+      if (i < L) {
+        const double vi = v[i];
+        for (int cc = wv; cc < CB; cc += CT / 64) {
+          const int c = tj * CB + cc;
+          if (c >= L) break;
+          Z[size_t(j0 + c) * n + j0 + i] -= scol[cc] * vi;
+        }
+      }
+      __syncthreads();
+    }
+    grid.sync();
   }
 }
 
@@ -581,7 +774,8 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   // The kernels reduce every matrix to tridiagonal form and build Q; the
   // host runs the QL recurrences; k_syev_apply applies the recorded
   // rotations and writes the sorted eigenpairs.  Reduction size classes:
-  // LDS-resident up to 64 and up to 128, else the global-memory kernel.
+  // LDS-resident up to 64 and up to 128, else one cooperative launch per
+  // matrix over the whole device.
   // Workspace: every (d, e) first (one copy to the host), then every Q.
   std::vector<EigTask> t64, t128, tg;
   int64_t desize = 0, qsize = 0;
@@ -629,7 +823,50 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
     if (!t128.empty())
       hipLaunchKernelGGL(k_syev_lds<128>, dim3(unsigned(t128.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
     q += t128.size();
-    if (!tg.empty()) hipLaunchKernelGGL(k_syev, dim3(unsigned(tg.size())), dim3(ET), 0, ctx->stream, q, Abase, Wbase, Z);
+    // above 128: side by side, one workgroup each (time ~ the largest
+    // n^3), or one after the other, each a cooperative launch over the
+    // whole device (time ~ the sum of n^2) -- whichever the model of the
+    // measured costs (n = 256: 14 ms / 8.4 ms) says is shorter
+    double t_side = 0, t_coop = 0;
+    for (const EigTask &t : tg) {
+      const double r = double(t.n) / 256.0;
+      t_side = std::max(t_side, 14.0 * r * r * r);
+      t_coop += 8.4 * r * r;
+    }
+    if (!tg.empty() && t_side < t_coop) {
+      hipLaunchKernelGGL(k_syev, dim3(unsigned(tg.size())), dim3(ET), 0, ctx->stream, q, Abase, Wbase, Z);
+      tg.clear();
+    }
+    static int coop_blocks = 0;
+    if (!tg.empty() && coop_blocks == 0) {
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(k_syev_coop), CT, 0) !=
+              hipSuccess ||
+          occ < 1)
+        occ = 1;
+      coop_blocks = std::max(1, ctx->num_cus * std::min(occ, 2));
+    }
+    for (size_t b = all.size() - tg.size(); b < all.size(); ++b) {  // (offsets as uploaded)
+      const EigTask &t = all[b];
+      const int n = t.n, nt = (n + CB - 1) / CB;
+      double *wk = nullptr;
+      int rc2 = pool_alloc(ctx, sizeof(double) * coop_ws(n), reinterpret_cast<void **>(&wk));
+      if (rc2) {
+        pool_free(ctx, ws);
+        return rc2;
+      }
+      double *Ap = Abase + t.a_off, *Zp = Z + t.z_off, *DEp = Z + t.de_off;
+      int nn = n;
+      void *args[] = {&Ap, &nn, &Zp, &DEp, &wk};
+      const unsigned G = unsigned(std::min<long>(coop_blocks, long(nt) * nt));
+      hipError_t e2 = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_syev_coop), dim3(G), dim3(CT), args,
+                                                 0, ctx->stream);
+      pool_free(ctx, wk);  // stream-ordered
+      if (e2 != hipSuccess) {
+        pool_free(ctx, ws);
+        return hip_err(ctx, e2, "itx_syev (cooperative reduction)");
+      }
+    }
   }
   hipError_t e = hipGetLastError();
   // the tridiagonals of every matrix to the host (one wait)
