@@ -348,7 +348,7 @@ __global__ __launch_bounds__(CT) void k_syev_coop(double *__restrict__ A, int n,
 // from the stored reflectors (backward accumulation, the row and column of
 // each new trailing index reset to the unit vector), written out with
 // (d, e) for the host.
-constexpr int ES = 256;
+constexpr int ES = 512;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -592,8 +592,15 @@ constexpr int RB = 1024;  // rotations staged in LDS per batch
 // loads its AK positions together ahead of its rotations; the updated
 // element i travels to the next rotation in a register and each rotation
 // stores position i+1, which the chain does not touch again.
+template <int LDC>  // LDC > 0: Q rows in LDS with that leading dimension (32-bit offsets)
 __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const RotMeta &mt,
                                              const double2 *__restrict__ R, double2 *Rs) {
+  auto at = [&](int p) -> double & {
+    if constexpr (LDC > 0)
+      return zc[p * LDC];
+    else
+      return zc[size_t(p) * ld];
+  };
   int segrem = 0, i = 0;
   double carry = 0;
   for (int b0 = 0; b0 < mt.nrot; b0 += RB) {
@@ -614,7 +621,7 @@ __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const Ro
         const double2 h = Rs[q++];
         i = int(h.x);
         segrem = int(h.y);
-        carry = zc[size_t(i + 1) * ld];
+        carry = at(i + 1);
         continue;  // the header may end the batch
       }
       const int kk = min(AK, min(segrem, qn - q));
@@ -624,7 +631,7 @@ __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const Ro
       double2 g[AK];
 #pragma unroll
       for (int t = 0; t < AK; ++t) {
-        z[t] = zc[size_t(max(i - t, 0)) * ld];
+        z[t] = at(max(i - t, 0));
         // past kk the pair (c, s) = (0, -1) passes the carry through
         // exactly and its store goes to the dummy column, so the chain
         // from one carry to the next is a single FMA
@@ -637,12 +644,12 @@ __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const Ro
         const double cz = g[t].x * z[t];
         const double up = fma(g[t].y, z[t], g[t].x * carry);
         carry = fma(-g[t].y, carry, cz);
-        zc[size_t(on ? i - t + 1 : n) * ld] = up;
+        at(on ? i - t + 1 : n) = up;
       }
       q += kk;
       segrem -= kk;
       i -= kk;
-      if (segrem == 0) zc[size_t(i + 1) * ld] = carry;
+      if (segrem == 0) at(i + 1) = carry;
     }
   }
 }
@@ -683,7 +690,7 @@ __global__ __launch_bounds__(AT) void k_syev_apply(const RotMeta *__restrict__ m
     zc = Zbase + mt.z_off + r;
     ld = n;
   }
-  apply_stream(zc, ld, n, mt, R, Rs);
+  apply_stream<(LC > 0 ? AT : 0)>(zc, ld, n, mt, R, Rs);
   // ranks to LDS (the rotation buffer is free now), then the sorted output
   const double *ev = evr + mt.ev_off, *rk = ev + n;
   int *rks = reinterpret_cast<int *>(Rs);
