@@ -1,7 +1,9 @@
 // Args, QN, Index and block bookkeeping of the host layer (all integer /
 // host-side work, as in the reference).
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <map>
 #include <sstream>
 
 #include "itensor.h"
@@ -201,6 +203,11 @@ QN calcDiv(IndexSet const &is, Block const &b) {
   return q;
 }
 
+// Every block of flux div in block order.  The charges are summed as small
+// integer vectors (one entry per charge name): the first r-1 indices are
+// walked as an odometer and the sectors of the last index with the missing
+// charge are looked up, so the cost is the number of combinations of the
+// first r-1 indices rather than of all r.
 std::pair<BlockOffsets, int64_t> getBlockOffsets(IndexSet const &is, QN const &div) {
   BlockOffsets bofs;
   if (is.empty()) {
@@ -208,21 +215,59 @@ std::pair<BlockOffsets, int64_t> getBlockOffsets(IndexSet const &is, QN const &d
     return {bofs, 1};
   }
   const int r = int(is.size());
-  Block I(r, 0);
-  int64_t total = 0;
+  std::vector<std::string> names;
+  auto addNames = [&](QN const &q) {
+    for (auto const &p : q.store()) names.push_back(p.first);
+  };
+  addNames(div);
+  for (auto const &I : is)
+    for (long b = 0; b < I.nblock(); ++b)
+      if (I.hasQNs()) addNames(I.qn0(b));
+  std::sort(names.begin(), names.end());
+  names.erase(std::unique(names.begin(), names.end()), names.end());
+  using Charge = std::array<int, 4>;
+  if (names.size() > 4) throw ITError("getBlockOffsets: more than four charge names");
+  auto charge = [&](QN const &q, int sign) {
+    Charge c{0, 0, 0, 0};
+    for (auto const &p : q.store())
+      c[size_t(std::lower_bound(names.begin(), names.end(), p.first) - names.begin())] = sign * p.second;
+    return c;
+  };
+  std::vector<std::vector<Charge>> ch{size_t(r)};
+  for (int j = 0; j < r; ++j) {
+    Index const &I = is[size_t(j)];
+    for (long b = 0; b < I.nblock(); ++b) ch[size_t(j)].push_back(I.hasQNs() ? charge(I.qn0(b), int(I.dir())) : Charge{});
+  }
+  std::map<Charge, std::vector<int>> last;
+  for (int b = 0; b < int(ch[size_t(r - 1)].size()); ++b) last[ch[size_t(r - 1)][size_t(b)]].push_back(b);
+  const Charge target = charge(div, 1);
+  std::vector<Block> blocks;
+  Block I(size_t(r), 0);
   while (true) {
-    if (calcDiv(is, I) == div) {
-      int64_t d = 1;
-      for (int j = 0; j < r; ++j) d *= is[j].blocksize0(I[j]);
-      bofs.push_back({I, total});
-      total += d;
-    }
+    Charge need = target;
+    for (int j = 0; j + 1 < r; ++j)
+      for (int k = 0; k < 4; ++k) need[size_t(k)] -= ch[size_t(j)][size_t(I[size_t(j)])][size_t(k)];
+    auto it = last.find(need);
+    if (it != last.end())
+      for (int b : it->second) {
+        I[size_t(r - 1)] = b;
+        blocks.push_back(I);
+      }
     int j = 0;  // first index fastest
-    for (; j < r; ++j) {
-      if (++I[j] < is[j].nblock()) break;
-      I[j] = 0;
+    for (; j + 1 < r; ++j) {
+      if (++I[size_t(j)] < is[size_t(j)].nblock()) break;
+      I[size_t(j)] = 0;
     }
-    if (j == r) break;
+    if (j + 1 >= r) break;
+  }
+  std::sort(blocks.begin(), blocks.end(), blockLess);
+  int64_t total = 0;
+  bofs.reserve(blocks.size());
+  for (auto const &b : blocks) {
+    int64_t d = 1;
+    for (int j = 0; j < r; ++j) d *= is[size_t(j)].blocksize0(b[size_t(j)]);
+    bofs.push_back({b, total});
+    total += d;
   }
   return {bofs, total};
 }

@@ -469,7 +469,8 @@ bool sameOffsets(BlockOffsets const &a, BlockOffsets const &b) {
 }  // namespace
 
 // ------------------------------------------------------------ ContractPlan --
-ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool perBlock) : perBlock_(perBlock) {
+ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool perBlock, Storage const *cover,
+                           std::vector<int> const *cperm) : perBlock_(perBlock) {
   if (!A || !B) throw ITError("ContractPlan: default constructed ITensor");
   if (A.isCombiner() || B.isCombiner()) throw ITError("ContractPlan: combiner operand");
   if (A.isQN() != B.isQN()) throw ITError("ContractPlan: QN and dense operands");
@@ -490,6 +491,43 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
   }
   QNTaskList tl = qnTaskList(A.inds(), A.store().offsets, B.inds(), B.store().offsets);
   N_ = tl.N;
+  if (cover && (cperm || cover->offsets.size() > tl.coffs.size())) {
+    // re-address every task's C block in the covering layout (result
+    // indices in the order cperm names); a produced block missing from it
+    // leaves the natural layout in place
+    const int rC = int(tl.N.size());
+    std::vector<int64_t> noff(tl.coffs.size());
+    bool ok = !cperm || int(cperm->size()) == rC;
+    Block pb(size_t(rC), 0);
+    for (size_t c = 0; c < tl.coffs.size() && ok; ++c) {
+      Block const &nb = tl.coffs[c].block;
+      for (int q = 0; q < rC; ++q) pb[size_t(q)] = nb[size_t(cperm ? (*cperm)[size_t(q)] : q)];
+      noff[c] = offsetOf(cover->offsets, pb);
+      ok = noff[c] >= 0;
+    }
+    if (ok) {
+      for (int t = 0; t < tl.nt; ++t) {
+        int64_t &co = tl.offs[3 * size_t(t) + 2];
+        auto it = std::lower_bound(tl.coffs.begin(), tl.coffs.end(), co,
+                                   [](BlOf const &x, int64_t v) { return x.offset < v; });
+        co = noff[size_t(it - tl.coffs.begin())];
+      }
+      if (cperm) {
+        IndexSet N2;
+        std::vector<int32_t> lc2;
+        for (int q = 0; q < rC; ++q) {
+          N2.push_back(tl.N[size_t((*cperm)[size_t(q)])]);
+          lc2.push_back(tl.lc[size_t((*cperm)[size_t(q)])]);
+        }
+        tl.N = N2;
+        tl.lc = lc2;
+        N_ = tl.N;
+      }
+      zeroOut_ = cover->offsets.size() > tl.coffs.size();
+      tl.coffs = cover->offsets;
+      tl.csize = int64_t(cover->size);
+    }
+  }
   coffs_ = tl.coffs;
   size_ = size_t(tl.csize);
   const int rA = A.order(), rB = B.order(), rC = int(N_.size());
@@ -524,7 +562,7 @@ ContractPlan::~ContractPlan() {
 }
 
 std::shared_ptr<Storage> ContractPlan::newOutput() const {
-  return qn_ ? mkQDense(coffs_, size_, false) : mkDense(size_);
+  return qn_ ? mkQDense(coffs_, size_, zeroOut_) : mkDense(size_);
 }
 
 void ContractPlan::exec(double const *A, double const *B, double *C) const {

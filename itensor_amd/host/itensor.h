@@ -355,7 +355,13 @@ QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet c
 // reference's per-block loop -- instead of one grouped launch.
 class ContractPlan {
  public:
-  ContractPlan(ITensor const &A, ITensor const &B, int prec, bool perBlock = false);
+  // cover (QN only, optional): a block layout of the result holding every
+  // block the contraction produces; the result is then laid out as cover,
+  // with the blocks no block pair reaches left at zero.  cperm (optional,
+  // with cover): result index q of that layout is index cperm[q] of the
+  // natural result order (the result is written in the permuted order).
+  ContractPlan(ITensor const &A, ITensor const &B, int prec, bool perBlock = false,
+               Storage const *cover = nullptr, std::vector<int> const *cperm = nullptr);
   ~ContractPlan();
   ContractPlan(ContractPlan const &) = delete;
   ContractPlan &operator=(ContractPlan const &) = delete;
@@ -373,7 +379,7 @@ class ContractPlan {
 
  private:
   IndexSet N_;
-  bool qn_ = false, perBlock_ = false;
+  bool qn_ = false, perBlock_ = false, zeroOut_ = false;
   BlockOffsets coffs_;
   size_t size_ = 0;
   double flops_ = 0;

@@ -167,7 +167,21 @@ LocalOp::Program *LocalOp::program(ITensor const &phi) const {
   // structure-only stand-in for each stage's left operand
   ITensor cur = phi;
   for (size_t i = 0; i < ops.size(); ++i) {
-    prg->stages.push_back(std::make_unique<ContractPlan>(cur, *ops[i], prec_, mode_ == HpsiMode::PerBlock));
+    Storage const *cover = nullptr;
+    std::vector<int> cperm;
+    if (cover_ && prg->qn && i + 1 == ops.size() && mode_ != HpsiMode::PerBlock) {
+      // result of the chain in phi's index order and block layout
+      IndexSet o = productInds(cur.inds(), ops[i]->inds());
+      for (auto &I : o) I.noPrime();
+      if (o.size() == phi.inds().size()) {
+        for (auto const &I : phi.inds()) cperm.push_back(indexPosition(o, I));
+        bool ok = true;
+        for (size_t q = 0; q < cperm.size(); ++q) ok = ok && cperm[q] >= 0 && o[size_t(cperm[q])].dir() == phi.inds()[q].dir();
+        if (ok) cover = &phi.store();
+      }
+    }
+    prg->stages.push_back(std::make_unique<ContractPlan>(cur, *ops[i], prec_, mode_ == HpsiMode::PerBlock, cover,
+                                                         cover ? &cperm : nullptr));
     ContractPlan const &cp = *prg->stages.back();
     prg->flops += cp.flops();
     auto st = std::make_shared<Storage>();
@@ -329,6 +343,13 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   PH.setPrecision(int(args.getInt("HpsiPrecision", ITX_PREC_F64)));
   PH.setFusion(args.getBool("FuseMPO", true));
   PH.setMode(HpsiMode(args.getInt("HpsiMode", long(HpsiMode::Graph))));
+  // QN: davidson starts from phi widened to every flux-allowed block
+  // (zeros where psi(b)*psi(b+1) has none) and the products keep that
+  // layout, so the Krylov vectors of a bond share one block set.  The
+  // values are those of the reference's growing block sets; the added
+  // blocks are exact zeros, whose density-matrix weight is zero.
+  const bool fullBlocks = args.getBool("FullBlockKrylov", true);
+  PH.setCoverOutput(fullBlocks);
   const bool syncTiming = args.getBool("SyncTiming", false);
   const long finalFP64 = args.getInt("FinalFP64Sweeps", 0);
   Real energy = NAN;
@@ -355,6 +376,11 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
       PH.position(b, psi);
       auto t1 = mark();
       ITensor phi = psi(b) * psi(b + 1);
+      if (fullBlocks && phi.isQN()) {
+        ITensor full = qdenseZeros(phi.inds(), div(phi));
+        full += phi;
+        phi = full;
+      }
       PH.fuseOps(phi.inds());
       DavidsonStats st;
       energy = davidson(PH, phi, args, &st);

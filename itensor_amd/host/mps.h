@@ -77,6 +77,15 @@ class LocalOp {
     mode_ = m;
   }
   HpsiMode mode() const { return mode_; }
+  // Full block set: when phi (QN) holds every block its product can reach
+  // and the product has phi's index order, the product is laid out with
+  // phi's blocks (those no block pair reaches are zero), so that the
+  // Krylov vectors of davidson share one structure -- one planned chain
+  // and aligned vector operations per bond instead of one per new block set
+  void setCoverOutput(bool on) {
+    if (on != cover_) progs_.clear();
+    cover_ = on;
+  }
   // Use W12 = Op1*Op2 in the bulk case when ((phi*L)*W12)*R yields exactly
   // the index order of ((phi*L)*Op1)*Op2*R; returns whether it does.
   bool setFused(ITensor const *W12, IndexSet const &phiInds);
@@ -97,6 +106,7 @@ class LocalOp {
   ITensor const *W12_ = nullptr;
   int prec_ = ITX_PREC_F64;
   HpsiMode mode_ = HpsiMode::Graph;
+  bool cover_ = false;
   mutable std::vector<std::shared_ptr<Program>> progs_;  // most recent last
 };
 
@@ -111,6 +121,7 @@ class LocalMPO {
   ITensor const &R() const { return PH_.at(size_t(RHlim_)); }
   void setPrecision(int p) { lop_.setPrecision(p); }
   void setMode(HpsiMode m) { lop_.setMode(m); }
+  void setCoverOutput(bool on) { lop_.setCoverOutput(on); }
   double flops(ITensor const &phi) const { return lop_.flops(phi); }
   std::pair<ITensor, double> deltaRho(ITensor const &AA, ITensor const &combine, Direction dir) const {
     return lop_.deltaRho(AA, combine, dir);

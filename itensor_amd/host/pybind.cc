@@ -260,6 +260,7 @@ PYBIND11_MODULE(_itensor_host, m) {
            "deltaRho(AA, combiner, dir) (localop.h:375-406): (drho, trace)")
       .def("size", [](PyLocalOp &o) { return o.op.size(); })
       .def("set_precision", [](PyLocalOp &o, int p) { o.op.setPrecision(p); })
+      .def("set_cover_output", [](PyLocalOp &o, bool on) { o.op.setCoverOutput(on); })
       .def("set_mode", [](PyLocalOp &o, int m) {
         if (m < 0 || m > 3) throw ITError("LocalOp.set_mode: mode must be 0..3");
         o.op.setMode(HpsiMode(m));
@@ -269,6 +270,12 @@ PYBIND11_MODULE(_itensor_host, m) {
   m.attr("HPSI_PLANNED") = int(HpsiMode::Planned);
   m.attr("HPSI_GRAPH") = int(HpsiMode::Graph);
   m.attr("HPSI_PER_BLOCK") = int(HpsiMode::PerBlock);
+  m.def("block_offsets", [](std::vector<Index> const &is, QNList const &div) {
+    auto [offs, total] = getBlockOffsets(is, qnFrom(div));
+    std::vector<std::pair<std::vector<int>, int64_t>> r;
+    for (auto &b : offs) r.push_back({b.block, b.offset});
+    return std::make_pair(r, total);
+  }, py::arg("inds"), py::arg("div") = QNList(), "getBlockOffsets: the blocks of flux div in block order, host only");
   m.def("random_itensor", [](std::vector<Index> const &is, QNList const &div, uint64_t seed) {
     return randomITensor(qnFrom(div), is, seed);
   }, py::arg("inds"), py::arg("div") = QNList(), py::arg("seed") = 1);

@@ -421,3 +421,39 @@ def test_delta_rho_matches_numpy(H, fromleft):
     assert np.max(np.abs(D - D.T)) < 1e-12
     assert np.max(np.abs(np.sort(np.linalg.eigvalsh(D)) - np.sort(np.linalg.eigvalsh(ref)))) < 1e-10 * np.max(ref)
     assert abs(tr - np.trace(ref)) < 1e-10 * np.trace(ref)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("fuse", [False, True])
+def test_local_product_full_block_layout(H, mode, fuse):
+    """LocalOp::setCoverOutput: for a phi holding every flux-allowed block,
+    in any index order, the product comes out in phi's index order and
+    block layout, with the oracle's values (and exact zeros in the blocks
+    no block pair reaches)."""
+    N, b = 10, 5
+    sites = OD.spin_half_sites(N, True)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)])
+    _, psi = OD.dmrg(Hm, psi0, [{"MaxDim": 20, "Cutoff": 1e-12}] * 2)
+    PH = OD.LocalMPO(Hm)
+    PH.position(b, psi)
+    phi = psi(b) * psi(b + 1)
+    ref = PH.product(phi)
+    refD = ref.to_dense_array()
+    phiD = phi.to_dense_array()
+    div = [(n, v) for n, v in phi.div().q]
+    op = H.LocalOp(cten(Hm[b - 1]), cten(Hm[b]), cten(PH.PH[b - 1]), cten(PH.PH[b + 2]), fuse)
+    op.set_mode(mode)
+    op.set_cover_output(True)
+    for perm in ([0, 1, 2, 3], [3, 2, 1, 0], [1, 3, 0, 2]):
+        inds = [phi.inds[p] for p in perm]
+        full = H.from_dense([cidx(i) for i in inds], np.transpose(phiD, perm), div)
+        assert len(full.offsets) >= len(phi.offsets)
+        for rep in range(2):
+            got = op.product(full)
+            assert same_inds(got, full)
+            assert [(tuple(x), o) for x, o in got.offsets] == [(tuple(x), o) for x, o in full.offsets]
+            # the oracle's result in phi's order
+            rperm = [[ (i.id, i.plev) for i in ref.inds].index((j.id, 0)) for j in inds]
+            want = np.transpose(refD, rperm)
+            assert _rel(oten(got).to_dense_array(), want) < 1e-12

@@ -160,3 +160,31 @@ def test_autompo_rejects_odd_terms():
 @pytest.mark.parametrize("Nx,Ny,yp", [(8, 4, True), (3, 2, True), (2, 3, True), (4, 4, False), (5, 1, False)])
 def test_square_lattice_matches_oracle(Nx, Ny, yp):
     assert [tuple(b) for b in H_.square_lattice(Nx, Ny, yp)] == OH.square_lattice(Nx, Ny, yp)
+
+
+# ---- block enumeration of the host layer (no device) -----------------------------
+
+def test_block_offsets_match_oracle():
+    """getBlockOffsets of the host layer (charge lookup on the last index)
+    against the oracle's exhaustive walk, on random index sets with one or
+    two charges, mixed arrows and orders 1-4."""
+    from oracle import itensor as OI
+    rng = np.random.default_rng(7)
+    names = [("Sz",), ("Nf", "Sz")]
+    for trial in range(60):
+        nm = names[trial % 2]
+        r = 1 + trial % 4
+        inds_o, inds_h = [], []
+        for k in range(r):
+            nsec = int(rng.integers(1, 6))
+            secs = [({n: int(rng.integers(-2, 3)) for n in nm}, int(rng.integers(1, 4))) for _ in range(nsec)]
+            d = int(rng.choice([-1, 1]))
+            io = OI.Index([(OI.QN(q), m) for q, m in secs], f"i{k}", dir=d)
+            inds_o.append(io)
+            qns = [([(n, v) for n, v in sorted(q.items()) if v], m) for q, m in secs]
+            inds_h.append(H_.Index(1000 + 10 * trial + k, sum(m for _, m in secs), d, 0, f"i{k}", qns))
+        div = {n: int(rng.integers(-2, 3)) for n in nm}
+        ref, rtot = OI.get_block_offsets(inds_o, OI.QN(div))
+        got, gtot = H_.block_offsets(inds_h, [(n, v) for n, v in sorted(div.items()) if v])
+        assert [tuple(b) for b, _ in got] == [tuple(b) for b, _ in ref], trial
+        assert [o for _, o in got] == [o for _, o in ref] and gtot == rtot, trial
