@@ -27,6 +27,7 @@
 #include <queue>
 #include <sstream>
 
+#include <unordered_map>
 #include "itx_internal.h"
 
 namespace itx {
@@ -1006,22 +1007,33 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   bool akc = false, bkc = false, kc_set = false;
 
   double flops = 0;
-  for (int t = 0; t < ntasks; ++t) {
-    const int64_t *dA = blkA + size_t(t) * rA;
-    const int64_t *dB = blkB + size_t(t) * rB;
-    int64_t dC[ITX_MAX_RANK], sA[ITX_MAX_RANK], sB[ITX_MAX_RANK], sC[ITX_MAX_RANK];
+  // Everything a task needs besides its offsets follows from the block
+  // dims of its A and B blocks; QN task lists repeat few distinct dims, so
+  // the analysis (GEMM digits, offset tables, extents) is done once per
+  // distinct (dims A, dims B) and looked up for the other tasks.
+  struct DimInfo {
+    int64_t dC[ITX_MAX_RANK];
+    int32_t M, N, K;
+    int64_t tAM, tCM, tBN, tCN, tAK, tBK;
+    int64_t am, cm, bn, cn, ak, bk;
+    bool akc, bkc, empty;
+  };
+  std::vector<DimInfo> infos;
+  std::vector<int64_t> ikeys;  // the dims of each entry (rA + rB per entry)
+  std::unordered_map<uint64_t, int> icache;
+  const int rAB = rA + rB;
+  auto analyze = [&](const int64_t *dA, const int64_t *dB, DimInfo &fi) -> int {
+    int64_t sA[ITX_MAX_RANK], sB[ITX_MAX_RANK], sC[ITX_MAX_RANK];
+    int64_t *dC = fi.dC;
     for (int i = 0; i < rA; ++i) {
       if (dA[i] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: negative dimension");
       if (lm.a2b[i] >= 0 && dB[lm.a2b[i]] != dA[i])
         return set_err(ctx, ITX_ERR_ARG, "contraction: contracted dimensions differ");
       if (lm.a2c[i] >= 0) dC[lm.a2c[i]] = dA[i];
     }
-    for (int j = 0; j < rB; ++j)
+    for (int j = 0; j < rB; ++j) {
+      if (dB[j] < 0) return set_err(ctx, ITX_ERR_ARG, "contraction: negative dimension");
       if (lm.b2c[j] >= 0) dC[lm.b2c[j]] = dB[j];
-    if (blkC) {
-      for (int c = 0; c < rC; ++c)
-        if (blkC[size_t(t) * rC + c] != dC[c])
-          return set_err(ctx, ITX_ERR_ARG, "contraction: dimensions of C do not match A and B");
     }
     colmajor_strides(rA, dA, sA);
     colmajor_strides(rB, dB, sB);
@@ -1039,7 +1051,11 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     const int64_t M = digits_size(dm), N = digits_size(dn), K = digits_size(dk);
     if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX)
       return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: GEMM extent exceeds 2^31");
-    if (M == 0 || N == 0) continue;  // empty C block: nothing to write
+    fi.M = int32_t(M);
+    fi.N = int32_t(N);
+    fi.K = int32_t(K);
+    fi.empty = (M == 0 || N == 0);  // empty C block: nothing to write
+    if (fi.empty) return ITX_OK;
     {
       int64_t na = 1, nb = 1, nc = 1;
       for (int i = 0; i < rA; ++i) na *= dA[i];
@@ -1048,47 +1064,115 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
       if (na > INT32_MAX || nb > INT32_MAX || nc > INT32_MAX)
         return set_err(ctx, ITX_ERR_UNSUPPORTED, "contraction: a block of 2^31 or more elements");
     }
+    fi.akc = !dk.empty() && dk[0].s1 == 1;
+    fi.bkc = !dk.empty() && dk[0].s2 == 1;
+    int64_t t0 = tcache.emit(dm, tab);
+    fi.tAM = t0;
+    fi.tCM = t0 + M;
+    t0 = tcache.emit(dn, tab);
+    fi.tBN = t0;
+    fi.tCN = t0 + N;
+    t0 = tcache.emit(dk, tab);
+    fi.tAK = t0;
+    fi.tBK = t0 + K;
+    digits_max(dm, fi.am, fi.cm);
+    digits_max(dn, fi.bn, fi.cn);
+    digits_max(dk, fi.ak, fi.bk);
+    return ITX_OK;
+  };
+  for (int t = 0; t < ntasks; ++t) {
+    const int64_t *dA = blkA + size_t(t) * rA;
+    const int64_t *dB = blkB + size_t(t) * rB;
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the dims
+    for (int i = 0; i < rA; ++i) h = (h ^ uint64_t(dA[i])) * 1099511628211ull;
+    for (int j = 0; j < rB; ++j) h = (h ^ uint64_t(dB[j])) * 1099511628211ull;
+    const DimInfo *fp = nullptr;
+    auto it = icache.find(h);
+    if (it != icache.end()) {
+      const int64_t *kd = ikeys.data() + size_t(it->second) * rAB;
+      if (std::equal(dA, dA + rA, kd) && std::equal(dB, dB + rB, kd + rA)) fp = &infos[size_t(it->second)];
+    }
+    DimInfo tmp;
+    if (!fp) {
+      int rc2 = analyze(dA, dB, tmp);
+      if (rc2) return rc2;
+      if (it == icache.end()) {  // first dims with this hash: remember them
+        icache.emplace(h, int(infos.size()));
+        ikeys.insert(ikeys.end(), dA, dA + rA);
+        ikeys.insert(ikeys.end(), dB, dB + rB);
+        infos.push_back(tmp);
+        fp = &infos.back();
+      } else {
+        fp = &tmp;
+      }
+    }
+    const DimInfo &fi = *fp;
+    if (blkC) {
+      for (int c = 0; c < rC; ++c)
+        if (blkC[size_t(t) * rC + c] != fi.dC[c])
+          return set_err(ctx, ITX_ERR_ARG, "contraction: dimensions of C do not match A and B");
+    }
+    if (fi.empty) continue;
     if (!kc_set) {
       kc_set = true;
-      akc = !dk.empty() && dk[0].s1 == 1;
-      bkc = !dk.empty() && dk[0].s2 == 1;
+      akc = fi.akc;
+      bkc = fi.bkc;
     }
     GemmTask g{};
     g.a_off = offs[3 * size_t(t)];
     g.b_off = offs[3 * size_t(t) + 1];
     g.c_off = offs[3 * size_t(t) + 2];
-    g.M = int32_t(M);
-    g.N = int32_t(N);
-    g.K = int32_t(K);
-    int64_t t0 = tcache.emit(dm, tab);
-    g.tAM = t0;
-    g.tCM = t0 + M;
-    t0 = tcache.emit(dn, tab);
-    g.tBN = t0;
-    g.tCN = t0 + N;
-    t0 = tcache.emit(dk, tab);
-    g.tAK = t0;
-    g.tBK = t0 + K;
-    {
-      int64_t am, cm, bn, cn, ak, bk;
-      digits_max(dm, am, cm);
-      digits_max(dn, bn, cn);
-      digits_max(dk, ak, bk);
-      extA = std::max(extA, g.a_off + am + ak + 1);
-      extB = std::max(extB, g.b_off + bk + bn + 1);
-      extC = std::max(extC, g.c_off + cm + cn + 1);
-    }
-    flops += 2.0 * double(M) * double(N) * double(K);
+    g.M = fi.M;
+    g.N = fi.N;
+    g.K = fi.K;
+    g.tAM = fi.tAM;
+    g.tCM = fi.tCM;
+    g.tBN = fi.tBN;
+    g.tCN = fi.tCN;
+    g.tAK = fi.tAK;
+    g.tBK = fi.tBK;
+    extA = std::max(extA, g.a_off + fi.am + fi.ak + 1);
+    extB = std::max(extB, g.b_off + fi.bk + fi.bn + 1);
+    extC = std::max(extC, g.c_off + fi.cm + fi.cn + 1);
+    flops += 2.0 * double(fi.M) * double(fi.N) * double(fi.K);
     tasks.push_back(g);
   }
 
-  // group tasks by C block (stable: keeps the planner's pair order)
-  std::vector<int> idx(tasks.size());
-  std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(),
-                   [&](int x, int y) { return tasks[x].c_off < tasks[y].c_off; });
+  // group tasks by C block, in C offset order and stable within a block
+  // (keeps the planner's pair order): a counting sort over the distinct
+  // C offsets
   std::vector<GemmTask> st(tasks.size());
-  for (size_t i = 0; i < idx.size(); ++i) st[i] = tasks[idx[i]];
+  {
+    // C offset -> bucket (first-appearance numbering) through an
+    // open-addressing table, then buckets renumbered in offset order
+    size_t cap = 16;
+    while (cap < 2 * tasks.size()) cap <<= 1;
+    std::vector<int64_t> hk(cap, -1);
+    std::vector<int> hv(cap, 0), bucket(tasks.size());
+    std::vector<int64_t> first;  // distinct C offsets, first-appearance order
+    for (size_t i = 0; i < tasks.size(); ++i) {
+      const int64_t c = tasks[i].c_off;
+      size_t h = size_t((uint64_t(c) * 0x9E3779B97F4A7C15ull) >> 20) & (cap - 1);
+      while (hk[h] != -1 && hk[h] != c) h = (h + 1) & (cap - 1);
+      if (hk[h] == -1) {
+        hk[h] = c;
+        hv[h] = int(first.size());
+        first.push_back(c);
+      }
+      bucket[i] = hv[h];
+    }
+    std::vector<int> ord(first.size()), rank(first.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return first[size_t(x)] < first[size_t(y)]; });
+    for (size_t r = 0; r < ord.size(); ++r) rank[size_t(ord[r])] = int(r);
+    std::vector<int> cnt(first.size() + 1, 0);
+    for (size_t i = 0; i < tasks.size(); ++i) {
+      bucket[i] = rank[size_t(bucket[i])];
+      ++cnt[size_t(bucket[i]) + 1];
+    }
+    for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+    for (size_t i = 0; i < tasks.size(); ++i) st[size_t(cnt[size_t(bucket[i])]++)] = tasks[i];
+  }
   struct Grp {
     int tb, te;
     int M, N;
@@ -1190,12 +1274,18 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     // heaviest units first (the persistent loop then balances like LPT);
     // stable so equal-cost tiles keep their m-major order
     {
-      std::vector<int> ui(units.size());
-      std::iota(ui.begin(), ui.end(), 0);
-      std::stable_sort(ui.begin(), ui.end(), [&](int x, int y) { return ucost[x] > ucost[y]; });
-      std::vector<WorkUnit> su(units.size());
-      for (size_t i = 0; i < ui.size(); ++i) su[i] = units[ui[i]];
-      units.swap(su);
+      std::vector<std::pair<double, int>> ui(units.size());
+      bool sorted = true;
+      for (size_t i = 0; i < units.size(); ++i) {
+        ui[i] = {-ucost[i], int(i)};  // (cost descending, position ascending): the stable order
+        sorted = sorted && (i == 0 || ucost[i] <= ucost[i - 1]);
+      }
+      if (!sorted) {
+        std::sort(ui.begin(), ui.end());
+        std::vector<WorkUnit> su(units.size());
+        for (size_t i = 0; i < ui.size(); ++i) su[i] = units[size_t(ui[i].second)];
+        units.swap(su);
+      }
     }
     auto *p = new itx_plan();
     p->prec = prec;

@@ -49,6 +49,8 @@ set_contract_precision = _h.set_contract_precision
 set_autotune = _h.set_autotune
 LocalOp = _h.LocalOp
 random_itensor = _h.random_itensor
+block_offsets = _h.block_offsets
+qn_task_list = _h.qn_task_list
 from_dense = _h.from_dense
 SiteSet = _h.SiteSet
 AutoMPO = _h.AutoMPO

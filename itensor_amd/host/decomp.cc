@@ -157,7 +157,10 @@ Spectrum diag_hermitian(ITensor H, ITensor &U, ITensor &D, Args const &args_in) 
   auto w = devAlloc(size_t(std::max<int64_t>(wtot, 1)));
   if (H.store().size)
     check(itx_memcpy_dtod(ctx(), W->p, H.store().data->p, sizeof(double) * H.store().size), "itx_memcpy_dtod");
-  if (nb) check(itx_syev_batched(ctx(), nb, ns.data(), aoffs.data(), W->p, woffs.data(), w->p), "itx_syev_batched");
+  if (nb) {
+    HostTimer ht(HT_SYEV);
+    check(itx_syev_batched(ctx(), nb, ns.data(), aoffs.data(), W->p, woffs.data(), w->p), "itx_syev_batched");
+  }
   std::vector<double> all(static_cast<size_t>(wtot));
   if (wtot) check(itx_memcpy_dtoh(ctx(), all.data(), w->p, sizeof(double) * size_t(wtot)), "itx_memcpy_dtoh");
   std::vector<double> probs = all;

@@ -26,7 +26,7 @@ struct TimerTable {
     static const char *names[HT_COUNT] = {"qnTaskList",    "itx_plan_qn",     "itx_plan_dense", "contract QN (eager)",
                                           "contract dense (eager)", "H*psi program build", "LocalOp::product",
                                           "norm / dot (with wait)", "diag_hermitian", "denmatDecomp",
-                                          "LocalMPO::position"};
+                                          "LocalMPO::position", "itx_chain_create", "itx_syev_batched"};
     std::fprintf(stderr, "[itx host timers]\n");
     for (int i = 0; i < HT_COUNT; ++i)
       if (calls[i]) std::fprintf(stderr, "  %-26s %9ld calls %10.3f ms\n", names[i], calls[i], secs[i] * 1e3);

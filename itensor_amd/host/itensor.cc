@@ -198,34 +198,53 @@ QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet c
     auto hi = std::upper_bound(lo, border.end(), 0, cmpBA);
     for (auto it = lo; it != hi; ++it) pairs.push_back({x, *it});
   }
-  // C blocks: sorted (reference order), deduplicated, prefix-summed sizes
-  Block cblk(rC, 0);
-  auto cblock_of = [&](Pair const &pr) {
-    Block const &ab = Aoffs[size_t(pr.a)].block, &bb = Boffs[size_t(pr.b)].block;
-    for (int ia = 0; ia < rA; ++ia)
-      if (AtoC[ia] != -1) cblk[AtoC[ia]] = ab[ia];
-    for (int ib = 0; ib < rB; ++ib)
-      if (BtoC[ib] != -1) cblk[BtoC[ib]] = bb[ib];
-  };
-  std::map<Block, int, bool (*)(Block const &, Block const &)> cid(blockLess);
-  for (auto const &pr : pairs) {
-    cblock_of(pr);
-    cid.try_emplace(cblk, 0);
+  // C blocks: sorted (reference order), deduplicated, prefix-summed sizes.
+  // A C block is coded as the mixed-radix number sum_c block[c] * R_c with
+  // R_c = nblock(N_0) ... nblock(N_{c-1}): the last index is the most
+  // significant digit, so numeric order is the reverse-lexicographic block
+  // order, and the code of a pair is the A part's plus the B part's.
+  std::vector<int64_t> R(size_t(rC) + 1, 1);
+  bool fits = true;
+  for (int c = 0; c < rC; ++c) {
+    const int64_t nb = N[c].nblock();
+    fits = fits && R[size_t(c)] <= (int64_t(1) << 62) / std::max<int64_t>(1, nb);
+    R[size_t(c) + 1] = fits ? R[size_t(c)] * nb : 0;
   }
+  if (!fits) throw ITError("contraction: too many result blocks to enumerate");
+  std::vector<int64_t> aCode(Aoffs.size(), 0), bCode(Boffs.size(), 0);
+  for (size_t x = 0; x < Aoffs.size(); ++x)
+    for (int ia = 0; ia < rA; ++ia)
+      if (AtoC[ia] != -1) aCode[x] += Aoffs[x].block[size_t(ia)] * R[size_t(AtoC[ia])];
+  for (size_t y = 0; y < Boffs.size(); ++y)
+    for (int ib = 0; ib < rB; ++ib)
+      if (BtoC[ib] != -1) bCode[y] += Boffs[y].block[size_t(ib)] * R[size_t(BtoC[ib])];
+  const int nt = int(pairs.size());
+  std::vector<int64_t> pkey(pairs.size(), 0);
+  for (int t = 0; t < nt; ++t) pkey[size_t(t)] = aCode[size_t(pairs[size_t(t)].a)] + bCode[size_t(pairs[size_t(t)].b)];
+  std::vector<int64_t> ukey(pkey);
+  std::sort(ukey.begin(), ukey.end());
+  ukey.erase(std::unique(ukey.begin(), ukey.end()), ukey.end());
   {
-    int k = 0;
     int64_t cur = 0;
-    tl.coffs.reserve(cid.size());
-    for (auto &kv : cid) {
-      kv.second = k++;
+    tl.coffs.reserve(ukey.size());
+    for (int64_t code : ukey) {
+      Block cb(size_t(rC), 0);
       int64_t sz = 1;
-      for (int c = 0; c < rC; ++c) sz *= N[c].blocksize0(kv.first[c]);
-      tl.coffs.push_back({kv.first, cur});
+      for (int c = 0; c < rC; ++c) {
+        cb[size_t(c)] = int((code / R[size_t(c)]) % N[c].nblock());
+        sz *= N[c].blocksize0(cb[size_t(c)]);
+      }
+      tl.coffs.push_back({std::move(cb), cur});
       cur += sz;
     }
     tl.csize = cur;
   }
-  const int nt = int(pairs.size());
+  // block dims of every A and B block, once
+  std::vector<int64_t> adims(Aoffs.size() * size_t(rA)), bdims(Boffs.size() * size_t(rB));
+  for (size_t x = 0; x < Aoffs.size(); ++x)
+    for (int i = 0; i < rA; ++i) adims[x * rA + i] = Ais[i].blocksize0(Aoffs[x].block[size_t(i)]);
+  for (size_t x = 0; x < Boffs.size(); ++x)
+    for (int i = 0; i < rB; ++i) bdims[x * rB + i] = Bis[i].blocksize0(Boffs[x].block[size_t(i)]);
   tl.nt = nt;
   tl.offs.resize(3 * size_t(nt));
   tl.bda.resize(size_t(nt) * rA);
@@ -234,15 +253,13 @@ QNTaskList qnTaskList(IndexSet const &Ais, BlockOffsets const &Aoffs, IndexSet c
   std::vector<char> seen(tl.coffs.size(), 0);
   for (int t = 0; t < nt; ++t) {
     Pair const &pr = pairs[size_t(t)];
-    cblock_of(pr);
-    const int ci = cid.find(cblk)->second;
+    const int ci = int(std::lower_bound(ukey.begin(), ukey.end(), pkey[size_t(t)]) - ukey.begin());
     tl.offs[3 * size_t(t)] = Aoffs[size_t(pr.a)].offset;
     tl.offs[3 * size_t(t) + 1] = Boffs[size_t(pr.b)].offset;
     tl.offs[3 * size_t(t) + 2] = tl.coffs[size_t(ci)].offset;
     if (!seen[size_t(ci)]) tl.cfirst[size_t(t)] = seen[size_t(ci)] = 1;
-    Block const &ab = Aoffs[size_t(pr.a)].block, &bb = Boffs[size_t(pr.b)].block;
-    for (int i = 0; i < rA; ++i) tl.bda[size_t(t) * rA + i] = Ais[i].blocksize0(ab[i]);
-    for (int i = 0; i < rB; ++i) tl.bdb[size_t(t) * rB + i] = Bis[i].blocksize0(bb[i]);
+    std::copy_n(adims.begin() + ptrdiff_t(size_t(pr.a) * rA), rA, tl.bda.begin() + ptrdiff_t(size_t(t) * rA));
+    std::copy_n(bdims.begin() + ptrdiff_t(size_t(pr.b) * rB), rB, tl.bdb.begin() + ptrdiff_t(size_t(t) * rB));
   }
   return tl;
 }

@@ -321,6 +321,8 @@ enum HostTimerId {
   HT_DIAG_HERMITIAN,
   HT_DENMAT,
   HT_ENV,
+  HT_CHAIN_CREATE,
+  HT_SYEV,
   HT_COUNT
 };
 class HostTimer {

@@ -200,8 +200,10 @@ LocalOp::Program *LocalOp::program(ITensor const &phi) const {
     std::vector<itx_plan *> plans;
     for (auto &cp : prg->stages)
       if (cp->plan()) plans.push_back(cp->plan());
-    if (plans.size() == prg->stages.size())
+    if (plans.size() == prg->stages.size()) {
+      HostTimer hc(HT_CHAIN_CREATE);
       check(itx_chain_create(ctx(), int(plans.size()), plans.data(), &prg->chain), "itx_chain_create");
+    }
   }
   if (progs_.size() >= 4) progs_.erase(progs_.begin());
   progs_.push_back(prg);

@@ -270,6 +270,24 @@ PYBIND11_MODULE(_itensor_host, m) {
   m.attr("HPSI_PLANNED") = int(HpsiMode::Planned);
   m.attr("HPSI_GRAPH") = int(HpsiMode::Graph);
   m.attr("HPSI_PER_BLOCK") = int(HpsiMode::PerBlock);
+  m.def("qn_task_list", [](std::vector<Index> const &Ais, std::vector<std::pair<std::vector<int>, int64_t>> const &Ao,
+                           std::vector<Index> const &Bis, std::vector<std::pair<std::vector<int>, int64_t>> const &Bo) {
+    BlockOffsets A, B;
+    for (auto &x : Ao) A.push_back({x.first, x.second});
+    for (auto &x : Bo) B.push_back({x.first, x.second});
+    QNTaskList tl = qnTaskList(Ais, A, Bis, B);
+    std::vector<std::pair<std::vector<int>, int64_t>> c;
+    for (auto &b : tl.coffs) c.push_back({b.block, b.offset});
+    py::dict d;
+    d["inds"] = tl.N;
+    d["coffs"] = c;
+    d["csize"] = tl.csize;
+    d["offs"] = tl.offs;
+    d["cfirst"] = tl.cfirst;
+    d["bda"] = tl.bda;
+    d["bdb"] = tl.bdb;
+    return d;
+  }, "the block-pair task list of a QN contraction (structure only, host only)");
   m.def("block_offsets", [](std::vector<Index> const &is, QNList const &div) {
     auto [offs, total] = getBlockOffsets(is, qnFrom(div));
     std::vector<std::pair<std::vector<int>, int64_t>> r;
