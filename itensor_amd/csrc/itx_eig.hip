@@ -11,6 +11,7 @@
 // rotations are applied to Q.  The work is split the way hybrid symmetric
 // eigensolvers split it: the O(n^3) parts on the device (reduction and Q:
 // one workgroup per matrix with the matrix resident in LDS up to 128, one
+// 16-wave workgroup per matrix over global memory up to 512, one
 // cooperative launch over the whole device per larger matrix; the
 // rotations applied to Q's rows in parallel in a second launch),
 // the O(n^2) scalar QL recurrence -- a chain of dependent flops that one
@@ -153,6 +154,222 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
   for (int i = tid; i < n; i += ET) {
     DE[i] = d[i];
     DE[n + i] = e[i];
+  }
+}
+
+// ---- one workgroup of 16 waves per matrix, 128 < n <= 64 RA -------------
+// The matrix stays in global memory (it fits the L2); the reduction makes
+// one pass over the trailing block per Householder step.  Step k: wave 0
+// brings column k up to date with the pending rank-2 update of step k-1
+// (v', w'), forms the reflector (v, tau) from it and stores v in the
+// column; then every thread walks its part of the trailing block
+// [k+1, n)^2 -- rows lane + 64 a, columns = wave (mod 16) -- applying the
+// pending update, storing the element and accumulating p = S v with the
+// updated value; the 16 partial sums of each row go through LDS, and wave 0
+// forms K = tau/2 v.p and w = p - K v, the next pending update.  Each
+// element is read and written once per step and every thread has up to
+// RA x (its columns) independent loads in flight.  Q is accumulated
+// backwards with the same column ownership: a wave owns whole columns of
+// Q, so the reflections need no barrier between waves.
+constexpr int MT = 1024, MW = MT / 64;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int RA, int UP, int UQ>
+__global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
+                                                 double *__restrict__ Zbase) {
+  constexpr int NM = 64 * RA;
+  __shared__ double ppart[MW][NM];
+  __shared__ double sv[NM], sw[NM], svp[NM], swp[NM];
+  __shared__ double sd[NM], se[NM], stau[NM];
+
+  const EigTask tk = tasks[blockIdx.x];
+  const int n = tk.n;
+  if (n <= 0 || n > NM) return;
+  double *__restrict__ A = Abase + tk.a_off;
+  double *__restrict__ Z = Zbase + tk.z_off;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < NM; i += MT) {
+    svp[i] = 0.0;
+    swp[i] = 0.0;
+  }
+  __syncthreads();
+
+  for (int k = 0; k + 2 < n; ++k) {
+    if (wave == 0) {
+      // column k with the pending update; rows i = k + lane + 64 q
+      const double wpk = swp[k], vpk = svp[k];
+      double c[RA];
+      double sig = 0.0;
+#pragma unroll
+      for (int q = 0; q < RA; ++q) {
+        const int i = k + lane + 64 * q;
+        c[q] = (i < n) ? A[size_t(k) * n + i] - (svp[i] * wpk + swp[i] * vpk) : 0.0;
+        if (i > k + 1 && i < n) sig += c[q] * c[q];
+      }
+      sig = wsum(sig);
+      const double dk = __shfl(c[0], 0, 64), x0 = __shfl(c[0], 1, 64);
+      double tau = 0.0, alpha = x0, v0 = x0;
+      if (sig > 0.0) {
+        const double mu = sqrt(x0 * x0 + sig);
+        v0 = (x0 <= 0) ? x0 - mu : x0 + mu;
+        alpha = (x0 <= 0) ? mu : -mu;
+        tau = 2.0 / (v0 * v0 + sig);
+      }
+#pragma unroll
+      for (int q = 0; q < RA; ++q) {
+        const int i = k + lane + 64 * q;
+        if (i > k && i < n) {
+          const double vi = (tau == 0.0) ? 0.0 : ((i == k + 1) ? v0 : c[q]);
+          sv[i] = vi;
+          A[size_t(k) * n + i] = vi;
+        }
+      }
+      if (lane == 0) {
+        sd[k] = dk;
+        se[k] = alpha;
+        stau[k] = tau;
+      }
+    }
+    __syncthreads();
+    const double tau = stau[k];
+    const int r0 = k + 1;
+    double pacc[RA], vpi[RA], wpi[RA];
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      const int i = r0 + lane + 64 * a;
+      pacc[a] = 0.0;
+      vpi[a] = (i < n) ? svp[i] : 0.0;
+      wpi[a] = (i < n) ? swp[i] : 0.0;
+    }
+    // groups of UP columns: every load of a group is issued before its
+    // stores, so that a thread keeps UP x RA loads in flight
+    for (int j0 = r0 + wave; j0 < n; j0 += MW * UP) {
+      double x[UP][RA];
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const int j = j0 + MW * u;
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+          const int i = r0 + lane + 64 * a;
+          x[u][a] = (j < n && i < n) ? A[size_t(j) * n + i] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UP; ++u) {
+        const int j = j0 + MW * u;
+        if (j < n) {
+          const double vj = sv[j], vpj = svp[j], wpj = swp[j];
+#pragma unroll
+          for (int a = 0; a < RA; ++a) {
+            const int i = r0 + lane + 64 * a;
+            if (i < n) {
+              const double y = x[u][a] - (vpi[a] * wpj + wpi[a] * vpj);
+              A[size_t(j) * n + i] = y;
+              pacc[a] += y * vj;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      const int i = r0 + lane + 64 * a;
+      if (i < n) ppart[wave][i] = pacc[a];
+    }
+    __syncthreads();
+    for (int i = r0 + tid; i < n; i += MT) {
+      double sm = 0.0;
+#pragma unroll
+      for (int w = 0; w < MW; ++w) sm += ppart[w][i];
+      sw[i] = tau * sm;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double sm = 0.0;
+#pragma unroll
+      for (int q = 0; q < RA; ++q) {
+        const int i = r0 + lane + 64 * q;
+        if (i < n) sm += sv[i] * sw[i];
+      }
+      const double K = 0.5 * tau * wsum(sm);
+#pragma unroll
+      for (int q = 0; q < RA; ++q) {
+        const int i = r0 + lane + 64 * q;
+        if (i < n) {
+          swp[i] = sw[i] - K * sv[i];
+          svp[i] = sv[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // the last 2 x 2 block with the pending update
+  if (tid == 0) {
+    if (n == 1) {
+      sd[0] = A[0];
+      se[0] = 0.0;
+    } else {
+      const int k = n - 2;
+      auto upd = [&](int i, int j) { return A[size_t(j) * n + i] - (svp[i] * swp[j] + swp[i] * svp[j]); };
+      sd[k] = upd(k, k);
+      se[k] = upd(k + 1, k);
+      sd[k + 1] = upd(k + 1, k + 1);
+      se[k + 1] = 0.0;
+    }
+  }
+  for (size_t x = tid; x < size_t(n) * n; x += MT) Z[x] = ((x / n) == (x % n)) ? 1.0 : 0.0;
+  __syncthreads();
+
+  // Q = H_0 ... H_{n-3}, backwards; wave w owns the columns c = w (mod 16)
+  for (int k = n - 3; k >= 0; --k) {
+    const double tau = stau[k];
+    if (tau == 0.0) continue;
+    const int r0 = k + 1;
+    double vi[RA];
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      const int i = r0 + lane + 64 * a;
+      vi[a] = (i < n) ? A[size_t(k) * n + i] : 0.0;
+    }
+    const int cstart = r0 + (((wave - r0) % MW) + MW) % MW;
+    for (int c0 = cstart; c0 < n; c0 += MW * UQ) {
+      double z[UQ][RA], sm[UQ];
+#pragma unroll
+      for (int u = 0; u < UQ; ++u) {
+        const int c = c0 + MW * u;
+        sm[u] = 0.0;
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+          const int i = r0 + lane + 64 * a;
+          z[u][a] = (c < n && i < n) ? Z[size_t(c) * n + i] : 0.0;
+          sm[u] += vi[a] * z[u][a];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UQ; ++u) sm[u] = tau * wsum(sm[u]);
+#pragma unroll
+      for (int u = 0; u < UQ; ++u) {
+        const int c = c0 + MW * u;
+        if (c < n) {
+#pragma unroll
+          for (int a = 0; a < RA; ++a) {
+            const int i = r0 + lane + 64 * a;
+            if (i < n) Z[size_t(c) * n + i] = z[u][a] - sm[u] * vi[a];
+          }
+        }
+      }
+    }
+  }
+  // (d, e) to the host, which runs the QL recurrence
+  double *DE = Zbase + tk.de_off;
+  for (int i = tid; i < n; i += MT) {
+    DE[i] = sd[i];
+    DE[n + i] = se[i];
   }
 }
 
@@ -781,10 +998,11 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   // The kernels reduce every matrix to tridiagonal form and build Q; the
   // host runs the QL recurrences; k_syev_apply applies the recorded
   // rotations and writes the sorted eigenpairs.  Reduction size classes:
-  // LDS-resident up to 64 and up to 128, else one cooperative launch per
+  // LDS-resident up to 64 and up to 128, one workgroup per matrix over
+  // global memory up to 256 and up to 512, else one cooperative launch per
   // matrix over the whole device.
   // Workspace: every (d, e) first (one copy to the host), then every Q.
-  std::vector<EigTask> t64, t128, tg;
+  std::vector<EigTask> t64, t128, t256, t512, tg;
   int64_t desize = 0, qsize = 0;
   for (int b = 0; b < nmat; ++b) {
     if (ns[b] < 0 || ns[b] > EMAX) return set_err(ctx, ITX_ERR_UNSUPPORTED, "itx_syev: n above 2048");
@@ -801,11 +1019,15 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
       t64.push_back(t);
     else if (ns[b] <= 128)
       t128.push_back(t);
+    else if (ns[b] <= 256)
+      t256.push_back(t);
+    else if (ns[b] <= 512)
+      t512.push_back(t);
     else
       tg.push_back(t);
   }
   std::vector<EigTask> all;
-  for (auto *v : {&t64, &t128, &tg}) all.insert(all.end(), v->begin(), v->end());
+  for (auto *v : {&t64, &t128, &t256, &t512, &tg}) all.insert(all.end(), v->begin(), v->end());
   const int ntask = int(all.size());
   if (ntask == 0) return ITX_OK;
   for (auto &t : all) t.z_off += desize;
@@ -830,7 +1052,13 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
     if (!t128.empty())
       hipLaunchKernelGGL(k_syev_lds<128>, dim3(unsigned(t128.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
     q += t128.size();
-    // above 128: side by side, one workgroup each (time ~ the largest
+    if (!t256.empty())
+      hipLaunchKernelGGL((k_syev_mid<4, 4, 4>), dim3(unsigned(t256.size())), dim3(MT), 0, ctx->stream, q, Abase, Z);
+    q += t256.size();
+    if (!t512.empty())
+      hipLaunchKernelGGL((k_syev_mid<8, 1, 2>), dim3(unsigned(t512.size())), dim3(MT), 0, ctx->stream, q, Abase, Z);
+    q += t512.size();
+    // above 512: side by side, one workgroup each (time ~ the largest
     // n^3), or one after the other, each a cooperative launch over the
     // whole device (time ~ the sum of n^2) -- whichever the model of the
     // measured costs (n = 256: 14 ms / 8.4 ms) says is shorter
