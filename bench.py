@@ -318,7 +318,7 @@ def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0):
             "hpsi_products_last_sweep": sum(r["nprod"] for r in last)}
 
 
-HUBBARD_MAXDIM = [50, 100, 200, 400]
+HUBBARD_MAXDIM = [50, 100, 200, 400, 800]
 
 
 def hubbard_leg(Nx: int, Ny: int, maxdims, prec: int, final_fp64: int = 0):
