@@ -457,3 +457,27 @@ def test_local_product_full_block_layout(H, mode, fuse):
             rperm = [[ (i.id, i.plev) for i in ref.inds].index((j.id, 0)) for j in inds]
             want = np.transpose(refD, rperm)
             assert _rel(oten(got).to_dense_array(), want) < 1e-12
+
+
+def test_full_block_krylov_same_trajectory(H):
+    """DMRG with the Krylov vectors on phi's full flux-allowed block set
+    (default) and with the reference's growing block sets (FullBlockKrylov
+    off) on the 4x2 Hubbard cylinder: the same energy after every bond step
+    to 1e-10 and the same bond dimensions and truncation errors; the added
+    blocks are exact zeros, so only the summation order of the grouped
+    contractions can differ."""
+    import bench
+
+    runs = []
+    for full in (True, False):
+        mpo, A = bench.hubbard_cylinder(H, 4, 2, 1.0, 4.0, True)
+        sw = [{"MaxDim": m, "Cutoff": 1e-10, "Niter": 3, "Noise": z}
+              for m, z in ((16, 1e-5), (32, 1e-6), (48, 0.0), (48, 0.0))]
+        runs.append(H.dmrg(mpo, A, sw, {"FullBlockKrylov": full}))
+    (E1, r1, _, _), (E2, r2, _, _) = runs
+    assert abs(E1 - E2) < 1e-10, (E1, E2)
+    assert len(r1) == len(r2)
+    for a, b in zip(r1, r2):
+        assert a["dim"] == b["dim"], (a, b)
+        assert abs(a["energy"] - b["energy"]) < 1e-10 * max(1.0, abs(b["energy"])), (a, b)
+        assert abs(a["truncerr"] - b["truncerr"]) < 1e-12 + 1e-6 * b["truncerr"], (a, b)
