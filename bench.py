@@ -318,7 +318,7 @@ def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0):
             "hpsi_products_last_sweep": sum(r["nprod"] for r in last)}
 
 
-HUBBARD_MAXDIM = [50, 100, 200, 400, 800]
+HUBBARD_MAXDIM = [50, 100, 200, 400, 400, 400, 400, 400]  # ramp, then converged at 400
 
 
 def hubbard_leg(Nx: int, Ny: int, maxdims, prec: int, final_fp64: int = 0):
@@ -396,7 +396,8 @@ def main():
                     help="H*psi forms of the DMRG leg ('+f64last': the last sweep in FP64)")
     ap.add_argument("--hubbard", default="8x4:" + ",".join(str(m) for m in HUBBARD_MAXDIM),
                     help="Hubbard cylinder leg: NxxNy:maxdim1,maxdim2,... ('' skips it)")
-    ap.add_argument("--hubbard-forms", default="f64,f32", help="H*psi forms of the Hubbard leg")
+    ap.add_argument("--hubbard-forms", default="f64,f32,bf16x3+f64last",
+                    help="H*psi forms of the Hubbard leg ('+f64last': the last sweep in FP64)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
