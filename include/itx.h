@@ -213,6 +213,14 @@ int itx_syev(itx_ctx *ctx, int n, double *A, double *w);
  * (hermitian.cc:206-257). */
 int itx_syev_batched(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
                      const int64_t *w_offs, double *w);
+/* As itx_syev_batched, and the same eigenvalues (bit for bit) also in host
+ * memory, w_host + w_offs[b], when the call returns -- while the
+ * eigenvectors are still being completed on the stream.  The truncation
+ * that follows the per-block diagonalisation in diagHermitian
+ * (hermitian.cc:259-307) needs only the eigenvalues, so the host can decide
+ * the kept dimensions without waiting for the eigenvector kernels. */
+int itx_syev_batched_host(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
+                          const int64_t *w_offs, double *w, double *w_host);
 
 /* ---- singular value decomposition (FP64) --------------------------------
  * A (m x n, column-major, device, not modified) = U diag(S) V^T with

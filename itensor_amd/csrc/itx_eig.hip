@@ -993,7 +993,7 @@ bool host_ql(int n, double *d, double *e, std::vector<double2> &rots) {
 }  // namespace
 
 int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, double *Abase,
-                 const int64_t *woffs, double *Wbase) {
+                 const int64_t *woffs, double *Wbase, double *whost) {
   if (nmat <= 0) return ITX_OK;
   // The kernels reduce every matrix to tridiagonal form and build Q; the
   // host runs the QL recurrences; k_syev_apply applies the recorded
@@ -1139,6 +1139,7 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
       int rank = 0;
       for (int j = 0; j < n; ++j) rank += (d[j] > d[c]) || (d[j] == d[c] && j < c);
       evr.push_back(double(rank));
+      if (whost) whost[t.w_off + rank] = d[c];  // the values the apply writes to w
     }
     const int k = n <= 64 ? 0 : n <= 128 ? 1 : n <= 256 ? 2 : 3;
     cls[k].push_back(mt);
@@ -1189,12 +1190,19 @@ extern "C" int itx_syev(itx_ctx *ctx, int n, double *A, double *w) {
   if (n == 0) return ITX_OK;
   const int32_t ns[1] = {n};
   const int64_t ao[1] = {0}, wo[1] = {0};
-  return itx::syev_batched(ctx, 1, ns, ao, A, wo, w);
+  return itx::syev_batched(ctx, 1, ns, ao, A, wo, w, nullptr);
 }
 
 extern "C" int itx_syev_batched(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
                                 const int64_t *w_offs, double *w) {
   if (!ctx || nmat < 0) return ITX_ERR_ARG;
   if (nmat == 0) return ITX_OK;
-  return itx::syev_batched(ctx, nmat, n, a_offs, A, w_offs, w);
+  return itx::syev_batched(ctx, nmat, n, a_offs, A, w_offs, w, nullptr);
+}
+
+extern "C" int itx_syev_batched_host(itx_ctx *ctx, int nmat, const int32_t *n, const int64_t *a_offs, double *A,
+                                     const int64_t *w_offs, double *w, double *w_host) {
+  if (!ctx || nmat < 0 || !w_host) return ITX_ERR_ARG;
+  if (nmat == 0) return ITX_OK;
+  return itx::syev_batched(ctx, nmat, n, a_offs, A, w_offs, w, w_host);
 }

@@ -157,12 +157,14 @@ Spectrum diag_hermitian(ITensor H, ITensor &U, ITensor &D, Args const &args_in) 
   auto w = devAlloc(size_t(std::max<int64_t>(wtot, 1)));
   if (H.store().size)
     check(itx_memcpy_dtod(ctx(), W->p, H.store().data->p, sizeof(double) * H.store().size), "itx_memcpy_dtod");
+  // eigenvalues straight to the host: the truncation below does not wait
+  // for the eigenvector kernels, which finish on the stream meanwhile
+  std::vector<double> all(static_cast<size_t>(wtot));
   if (nb) {
     HostTimer ht(HT_SYEV);
-    check(itx_syev_batched(ctx(), nb, ns.data(), aoffs.data(), W->p, woffs.data(), w->p), "itx_syev_batched");
+    check(itx_syev_batched_host(ctx(), nb, ns.data(), aoffs.data(), W->p, woffs.data(), w->p, all.data()),
+          "itx_syev_batched_host");
   }
-  std::vector<double> all(static_cast<size_t>(wtot));
-  if (wtot) check(itx_memcpy_dtoh(ctx(), all.data(), w->p, sizeof(double) * size_t(wtot)), "itx_memcpy_dtoh");
   std::vector<double> probs = all;
   std::sort(probs.begin(), probs.end(), std::greater<double>());
   Real truncerr = 0, docut_lower = -1, docut_upper = -1;

@@ -66,6 +66,7 @@ _PROTOS = {
     "itx_permute": (C.c_int, [_vp, _vp, _vp, C.c_int, _i64p, _i32p, C.c_double, C.c_double]),
     "itx_syev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "itx_syev_batched": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, _i64p, _vp]),
+    "itx_syev_batched_host": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, _i64p, _vp, _vp]),
     "itx_block_copy": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _i64p, C.c_double, C.c_double]),
     "itx_set_autotune": (C.c_int, [_vp, C.c_int]),
     "itx_tune_save": (C.c_int, [_vp, C.c_char_p]),
