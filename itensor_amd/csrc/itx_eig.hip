@@ -842,6 +842,37 @@ __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const Ro
         continue;  // the header may end the batch
       }
       const int kk = min(AK, min(segrem, qn - q));
+      if (kk == AK) {
+        // a full chunk: positions i .. i-AK+1 all belong to the sweep, so
+        // no clamp, mask or dummy column; the offsets from the chunk base
+        // are compile-time constants
+        double *zb = &at(i - (AK - 1));  // position i - t is zb[(AK - 1 - t) * step]
+        double z[AK];
+        double2 g[AK];
+#pragma unroll
+        for (int t = 0; t < AK; ++t) {
+          if constexpr (LDC > 0)
+            z[t] = zb[(AK - 1 - t) * LDC];
+          else
+            z[t] = zb[size_t(AK - 1 - t) * ld];
+          g[t] = Rs[q + t];
+        }
+#pragma unroll
+        for (int t = 0; t < AK; ++t) {
+          const double cz = g[t].x * z[t];
+          const double up = fma(g[t].y, z[t], g[t].x * carry);
+          carry = fma(-g[t].y, carry, cz);
+          if constexpr (LDC > 0)
+            zb[(AK - t) * LDC] = up;  // position i - t + 1
+          else
+            zb[size_t(AK - t) * ld] = up;
+        }
+        q += AK;
+        segrem -= AK;
+        i -= AK;
+        if (segrem == 0) at(i + 1) = carry;
+        continue;
+      }
       // all loads of the chunk ahead of its stores (the stores could alias
       // the rotation buffer as far as the compiler knows)
       double z[AK];
