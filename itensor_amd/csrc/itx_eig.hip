@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include <hip/hip_cooperative_groups.h>
 
@@ -36,7 +37,7 @@ constexpr int QL_MAXIT = 60;  // QL sweeps per eigenvalue before giving up
 struct EigTask {
   int32_t n, pad;
   int64_t a_off, w_off;  // A (in/out, col-major), w
-  int64_t z_off;         // workspace: Q, column-major n x (n+1) (the last column a dummy)
+  int64_t z_off;         // workspace: Q, column-major n x (n+1) (the last column unused)
   int64_t de_off;        // workspace: (d, e) handed to the host
 };
 
@@ -799,16 +800,17 @@ struct RotMeta {
   int32_t n, nrot;              // nrot: stream length (headers included)
 };
 constexpr int AT = 64;    // rows per apply block (one wave, one row per lane)
-constexpr int AK = 16;    // rotations per straight-line chunk
+constexpr int AK = 16;    // rotations per straight-line chunk (the longest)
 constexpr int RB = 1024;  // rotations staged in LDS per batch
 
 // Applies the rotation stream to this lane's row of Q (element (row, i) at
-// zc[i * ld]; column n is a dummy the padded tail of a chunk writes to).
-// Within a sweep position i is read once, before the rotation on (i, i+1),
-// and the rotation on (i+1, i+2) has left only i+1 changed, so a chunk
-// loads its AK positions together ahead of its rotations; the updated
+// zc[i * ld]).  Within a sweep position i is read once, before the rotation
+// on (i, i+1), and the rotation on (i+1, i+2) has left only i+1 changed, so
+// a chunk loads its positions together ahead of its rotations; the updated
 // element i travels to the next rotation in a register and each rotation
-// stores position i+1, which the chain does not touch again.
+// stores position i+1, which the chain does not touch again.  A sweep runs
+// as chunks of 16 rotations, its remainder as chunks of 8, 4, 2 and 1, so
+// no chunk reaches past the sweep and none needs a mask.
 template <int LDC>  // LDC > 0: Q rows in LDS with that leading dimension (32-bit offsets)
 __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const RotMeta &mt,
                                              const double2 *__restrict__ R, double2 *Rs) {
@@ -833,6 +835,36 @@ __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const Ro
     for (int u = 0; u < RB / AT; ++u) Rs[u * AT + threadIdx.x] = ld_[u];
     __syncthreads();
     int q = 0;
+    // C consecutive rotations of the current sweep, positions i .. i-C+1
+    // (all inside the sweep, so no clamp or mask): every load ahead of the
+    // stores, offsets from the chunk base compile-time constants
+    auto chunk = [&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      double *zb = &at(i - (C - 1));  // position i - t is zb[(C - 1 - t) * step]
+      double z[C];
+      double2 g[C];
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        if constexpr (LDC > 0)
+          z[t] = zb[(C - 1 - t) * LDC];
+        else
+          z[t] = zb[size_t(C - 1 - t) * ld];
+        g[t] = Rs[q + t];
+      }
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        const double cz = g[t].x * z[t];
+        const double up = fma(g[t].y, z[t], g[t].x * carry);
+        carry = fma(-g[t].y, carry, cz);
+        if constexpr (LDC > 0)
+          zb[(C - t) * LDC] = up;  // position i - t + 1
+        else
+          zb[size_t(C - t) * ld] = up;
+      }
+      q += C;
+      segrem -= C;
+      i -= C;
+    };
     while (q < qn) {
       if (segrem == 0) {  // a sweep header
         const double2 h = Rs[q++];
@@ -841,62 +873,18 @@ __device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const Ro
         carry = at(i + 1);
         continue;  // the header may end the batch
       }
-      const int kk = min(AK, min(segrem, qn - q));
-      if (kk == AK) {
-        // a full chunk: positions i .. i-AK+1 all belong to the sweep, so
-        // no clamp, mask or dummy column; the offsets from the chunk base
-        // are compile-time constants
-        double *zb = &at(i - (AK - 1));  // position i - t is zb[(AK - 1 - t) * step]
-        double z[AK];
-        double2 g[AK];
-#pragma unroll
-        for (int t = 0; t < AK; ++t) {
-          if constexpr (LDC > 0)
-            z[t] = zb[(AK - 1 - t) * LDC];
-          else
-            z[t] = zb[size_t(AK - 1 - t) * ld];
-          g[t] = Rs[q + t];
-        }
-#pragma unroll
-        for (int t = 0; t < AK; ++t) {
-          const double cz = g[t].x * z[t];
-          const double up = fma(g[t].y, z[t], g[t].x * carry);
-          carry = fma(-g[t].y, carry, cz);
-          if constexpr (LDC > 0)
-            zb[(AK - t) * LDC] = up;  // position i - t + 1
-          else
-            zb[size_t(AK - t) * ld] = up;
-        }
-        q += AK;
-        segrem -= AK;
-        i -= AK;
-        if (segrem == 0) at(i + 1) = carry;
-        continue;
-      }
-      // all loads of the chunk ahead of its stores (the stores could alias
-      // the rotation buffer as far as the compiler knows)
-      double z[AK];
-      double2 g[AK];
-#pragma unroll
-      for (int t = 0; t < AK; ++t) {
-        z[t] = at(max(i - t, 0));
-        // past kk the pair (c, s) = (0, -1) passes the carry through
-        // exactly and its store goes to the dummy column, so the chain
-        // from one carry to the next is a single FMA
-        const double2 gl = Rs[q + t];  // in the buffer's AK-entry tail when past qn
-        g[t] = make_double2(t < kk ? gl.x : 0.0, t < kk ? gl.y : -1.0);
-      }
-#pragma unroll
-      for (int t = 0; t < AK; ++t) {
-        const bool on = t < kk;
-        const double cz = g[t].x * z[t];
-        const double up = fma(g[t].y, z[t], g[t].x * carry);
-        carry = fma(-g[t].y, carry, cz);
-        at(on ? i - t + 1 : n) = up;
-      }
-      q += kk;
-      segrem -= kk;
-      i -= kk;
+      // the sweep's rotations in this batch, in chunks of 16, then 8, 4, 2, 1
+      const int avail = min(segrem, qn - q);
+      if (avail >= AK)
+        chunk(std::integral_constant<int, AK>{});
+      else if (avail >= 8)
+        chunk(std::integral_constant<int, 8>{});
+      else if (avail >= 4)
+        chunk(std::integral_constant<int, 4>{});
+      else if (avail >= 2)
+        chunk(std::integral_constant<int, 2>{});
+      else
+        chunk(std::integral_constant<int, 1>{});
       if (segrem == 0) at(i + 1) = carry;
     }
   }
