@@ -185,7 +185,7 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
                                                  double *__restrict__ Zbase) {
   constexpr int NM = 64 * RA;
   __shared__ double ppart[MW][NM];
-  __shared__ double sv[NM], sw[NM], svp[NM], swp[NM];
+  __shared__ double sv[NM], svp[NM], swp[NM];
   __shared__ double sd[NM], se[NM], stau[NM];
 
   const EigTask tk = tasks[blockIdx.x];
@@ -283,31 +283,34 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
       if (i < n) ppart[wave][i] = pacc[a];
     }
     __syncthreads();
-    for (int i = r0 + tid; i < n; i += MT) {
-      double sm = 0.0;
-#pragma unroll
-      for (int w = 0; w < MW; ++w) sm += ppart[w][i];
-      sw[i] = tau * sm;
-    }
-    __syncthreads();
+    // wave 0 alone sums the partials into p, forms K and the next pending
+    // update and goes on to the next step's column: everything it reads
+    // there it wrote itself, so the step needs no further barrier (the
+    // other waves read svp / swp / sv only after the next barrier)
     if (wave == 0) {
+      double pv[RA];
       double sm = 0.0;
 #pragma unroll
       for (int q = 0; q < RA; ++q) {
         const int i = r0 + lane + 64 * q;
-        if (i < n) sm += sv[i] * sw[i];
+        double ps = 0.0;
+        if (i < n) {
+#pragma unroll
+          for (int w = 0; w < MW; ++w) ps += ppart[w][i];
+        }
+        pv[q] = tau * ps;
+        if (i < n) sm += sv[i] * pv[q];
       }
       const double K = 0.5 * tau * wsum(sm);
 #pragma unroll
       for (int q = 0; q < RA; ++q) {
         const int i = r0 + lane + 64 * q;
         if (i < n) {
-          swp[i] = sw[i] - K * sv[i];
+          swp[i] = pv[q] - K * sv[i];
           svp[i] = sv[i];
         }
       }
     }
-    __syncthreads();
   }
   // the last 2 x 2 block with the pending update
   if (tid == 0) {
