@@ -180,7 +180,7 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-template <int RA, int UP, int UQ>
+template <int RA, int UP, int UQ, int QG>
 __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
                                                  double *__restrict__ Zbase) {
   constexpr int NM = 64 * RA;
@@ -329,33 +329,53 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
   for (size_t x = tid; x < size_t(n) * n; x += MT) Z[x] = ((x / n) == (x % n)) ? 1.0 : 0.0;
   __syncthreads();
 
-  // Q = H_0 ... H_{n-3}, backwards; wave w owns the columns c = w (mod 16)
-  for (int k = n - 3; k >= 0; --k) {
-    const double tau = stau[k];
-    if (tau == 0.0) continue;
-    const int r0 = k + 1;
-    double vi[RA];
+  // Q = H_0 ... H_{n-3}, backwards; wave w owns the columns c = w (mod 16).
+  // QG reflectors per pass: a column is loaded once, H_kh, ..., H_kl are
+  // applied to it in registers (each reflector masked to its rows >= k+1,
+  // so a column it does not reach gets an exact zero dot), stored once.
+  for (int kh = n - 3; kh >= 0; kh -= QG) {
+    const int kl = max(kh - QG + 1, 0);
+    const int r0 = kl + 1;
+    double vg[QG][RA], tg[QG];
 #pragma unroll
-    for (int a = 0; a < RA; ++a) {
-      const int i = r0 + lane + 64 * a;
-      vi[a] = (i < n) ? A[size_t(k) * n + i] : 0.0;
+    for (int g = 0; g < QG; ++g) {
+      const int k = kh - g;
+      tg[g] = (k >= kl) ? stau[k] : 0.0;
+#pragma unroll
+      for (int a = 0; a < RA; ++a) {
+        const int i = r0 + lane + 64 * a;
+        vg[g][a] = (k >= kl && i < n && i > k) ? A[size_t(k) * n + i] : 0.0;
+      }
     }
     const int cstart = r0 + (((wave - r0) % MW) + MW) % MW;
     for (int c0 = cstart; c0 < n; c0 += MW * UQ) {
-      double z[UQ][RA], sm[UQ];
+      double z[UQ][RA];
 #pragma unroll
       for (int u = 0; u < UQ; ++u) {
         const int c = c0 + MW * u;
-        sm[u] = 0.0;
 #pragma unroll
         for (int a = 0; a < RA; ++a) {
           const int i = r0 + lane + 64 * a;
           z[u][a] = (c < n && i < n) ? Z[size_t(c) * n + i] : 0.0;
-          sm[u] += vi[a] * z[u][a];
         }
       }
 #pragma unroll
-      for (int u = 0; u < UQ; ++u) sm[u] = tau * wsum(sm[u]);
+      for (int g = 0; g < QG; ++g) {
+        if (tg[g] == 0.0) continue;  // uniform
+        double sm[UQ];
+#pragma unroll
+        for (int u = 0; u < UQ; ++u) {
+          sm[u] = 0.0;
+#pragma unroll
+          for (int a = 0; a < RA; ++a) sm[u] += vg[g][a] * z[u][a];
+        }
+#pragma unroll
+        for (int u = 0; u < UQ; ++u) {
+          const double f = tg[g] * wsum(sm[u]);
+#pragma unroll
+          for (int a = 0; a < RA; ++a) z[u][a] -= f * vg[g][a];
+        }
+      }
 #pragma unroll
       for (int u = 0; u < UQ; ++u) {
         const int c = c0 + MW * u;
@@ -363,7 +383,7 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
 #pragma unroll
           for (int a = 0; a < RA; ++a) {
             const int i = r0 + lane + 64 * a;
-            if (i < n) Z[size_t(c) * n + i] = z[u][a] - sm[u] * vi[a];
+            if (i < n) Z[size_t(c) * n + i] = z[u][a];
           }
         }
       }
@@ -1075,10 +1095,10 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
       hipLaunchKernelGGL(k_syev_lds<128>, dim3(unsigned(t128.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
     q += t128.size();
     if (!t256.empty())
-      hipLaunchKernelGGL((k_syev_mid<4, 4, 4>), dim3(unsigned(t256.size())), dim3(MT), 0, ctx->stream, q, Abase, Z);
+      hipLaunchKernelGGL((k_syev_mid<4, 4, 2, 8>), dim3(unsigned(t256.size())), dim3(MT), 0, ctx->stream, q, Abase, Z);
     q += t256.size();
     if (!t512.empty())
-      hipLaunchKernelGGL((k_syev_mid<8, 1, 2>), dim3(unsigned(t512.size())), dim3(MT), 0, ctx->stream, q, Abase, Z);
+      hipLaunchKernelGGL((k_syev_mid<8, 1, 2, 4>), dim3(unsigned(t512.size())), dim3(MT), 0, ctx->stream, q, Abase, Z);
     q += t512.size();
     // above 512: side by side, one workgroup each (time ~ the largest
     // n^3), or one after the other, each a cooperative launch over the
