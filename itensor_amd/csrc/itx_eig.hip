@@ -165,7 +165,8 @@ __global__ __launch_bounds__(ET) void k_syev(const EigTask *__restrict__ tasks, 
 // (v', w'), forms the reflector (v, tau) from it and stores v in the
 // column; then every thread walks its part of the trailing block
 // [k+1, n)^2 -- rows lane + 64 a, columns = wave (mod 16) -- applying the
-// pending update, storing the element and accumulating p = S v with the
+// pending update, storing the element (column k+1 also to LDS, where wave 0
+// reads it for the next reflector) and accumulating p = S v with the
 // updated value; the 16 partial sums of each row go through LDS, and wave 0
 // forms K = tau/2 v.p and w = p - K v, the next pending update.  Each
 // element is read and written once per step and every thread has up to
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
   constexpr int NM = 64 * RA;
   __shared__ double ppart[MW][NM];
   __shared__ double sv[NM], svp[NM], swp[NM];
+  __shared__ double scol[NM];  // column k+1 as the pass of step k stored it
   __shared__ double sd[NM], se[NM], stau[NM];
 
   const EigTask tk = tasks[blockIdx.x];
@@ -209,7 +211,8 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
 #pragma unroll
       for (int q = 0; q < RA; ++q) {
         const int i = k + lane + 64 * q;
-        c[q] = (i < n) ? A[size_t(k) * n + i] - (svp[i] * wpk + swp[i] * vpk) : 0.0;
+        const double a0 = (i < n) ? (k == 0 ? A[size_t(k) * n + i] : scol[i]) : 0.0;
+        c[q] = (i < n) ? a0 - (svp[i] * wpk + swp[i] * vpk) : 0.0;
         if (i > k + 1 && i < n) sig += c[q] * c[q];
       }
       sig = wsum(sig);
@@ -271,6 +274,7 @@ __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tas
             if (i < n) {
               const double y = x[u][a] - (vpi[a] * wpj + wpi[a] * vpj);
               A[size_t(j) * n + i] = y;
+              if (j == r0) scol[i] = y;  // the next step's column, without a round trip
               pacc[a] += y * vj;
             }
           }
