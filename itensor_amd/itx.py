@@ -255,6 +255,18 @@ class Context:
         self.check(self.L.itx_syev_batched(self.h, len(ns), ns.ctypes.data_as(_i32p), ao.ctypes.data_as(_i64p),
                                            _vp(A), wo.ctypes.data_as(_i64p), _vp(w)), "itx_syev_batched")
 
+    def syev_batched_host(self, ns, a_offs, A: int, w_offs, w: int) -> np.ndarray:
+        """itx_syev_batched_host: as syev_batched, returning the eigenvalues
+        the call also hands to the host."""
+        ns = np.ascontiguousarray(ns, dtype=np.int32)
+        ao = np.ascontiguousarray(a_offs, dtype=np.int64)
+        wo = np.ascontiguousarray(w_offs, dtype=np.int64)
+        wh = np.zeros(int(max([o + n for o, n in zip(wo, ns)] + [1])))
+        self.check(self.L.itx_syev_batched_host(self.h, len(ns), ns.ctypes.data_as(_i32p), ao.ctypes.data_as(_i64p),
+                                                _vp(A), wo.ctypes.data_as(_i64p), _vp(w), _vp(wh.ctypes.data)),
+                   "itx_syev_batched_host")
+        return wh
+
     def gesvd(self, m: int, n: int, A: int, U: int, S: int, V: int):
         self.check(self.L.itx_gesvd(self.h, m, n, _vp(A), _vp(U), _vp(S), _vp(V)), "itx_gesvd")
 

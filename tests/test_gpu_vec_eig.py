@@ -121,6 +121,29 @@ def test_syev_batched_mixed_classes(ctx, dev):
         assert np.max(np.abs(U @ np.diag(wn) @ U.T - X)) / scale < 1e-12, n
 
 
+def test_syev_batched_host_eigenvalues(ctx, dev):
+    """itx_syev_batched_host hands the host the same eigenvalues (bit for
+    bit) that it writes to w, with the same eigenvectors as the plain
+    batched call."""
+    rng = np.random.default_rng(78)
+    ns = [40, 0, 100, 130, 260, 3]
+    mats = []
+    for n in ns:
+        X = rng.standard_normal((n, max(n // 3, 1)))
+        mats.append(X @ X.T)
+    a_offs = np.cumsum([0] + [n * n for n in ns])[:-1]
+    w_offs = np.cumsum([0] + ns)[:-1]
+    flat = np.concatenate([m.ravel(order="F") for m in mats] + [np.zeros(1)])
+    dA, dA2 = dev.put(flat), dev.put(flat)
+    dw, dw2 = dev.empty(sum(ns) + 1), dev.empty(sum(ns) + 1)
+    wh = ctx.syev_batched_host(ns, a_offs, dA.data_ptr(), w_offs, dw.data_ptr())
+    ctx.syev_batched(ns, a_offs, dA2.data_ptr(), w_offs, dw2.data_ptr())
+    w, w2 = dev.get(dw, (sum(ns) + 1,)), dev.get(dw2, (sum(ns) + 1,))
+    assert np.array_equal(wh[: sum(ns)], w[: sum(ns)])
+    assert np.array_equal(w[: sum(ns)], w2[: sum(ns)])
+    assert np.array_equal(dev.get(dA, (flat.size,)), dev.get(dA2, (flat.size,)))
+
+
 @pytest.mark.parametrize("m,n,kind", [(1, 1, "rand"), (1, 7, "rand"), (7, 1, "rand"), (5, 9, "rand"),
                                       (9, 5, "rand"), (40, 40, "rank"), (64, 130, "rand"), (130, 64, "rand"),
                                       (100, 100, "decay"), (33, 200, "decay")])
