@@ -99,7 +99,9 @@ __global__ __launch_bounds__(VT) void k_dot_host(VPtrs V, const double *__restri
     out[j] = do_sqrt ? sqrt(t) : t;
     cnt[j] = 0u;
     __threadfence_system();
-    if (atomicAdd(&cnt[MAXV], 1u) == unsigned(nv - 1)) {
+    if (nv == 1) {  // the only result is visible to the host: raise the flag
+      *reinterpret_cast<volatile unsigned long long *>(flag) = seq;
+    } else if (atomicAdd(&cnt[MAXV], 1u) == unsigned(nv - 1)) {
       cnt[MAXV] = 0u;
       __threadfence_system();
       *reinterpret_cast<volatile unsigned long long *>(flag) = seq;
