@@ -10,8 +10,7 @@
 // Reinsch & Wilkinson, Numer. Math. 11 (1968) 293, "tql2"), whose plane
 // rotations are applied to Q.  The work is split the way hybrid symmetric
 // eigensolvers split it: the O(n^3) parts on the device (reduction and Q:
-// one wave per matrix up to 64, one workgroup per matrix with the matrix
-// resident in LDS up to 128, one
+// one workgroup per matrix with the matrix resident in LDS up to 128, one
 // 16-wave workgroup per matrix over global memory up to 512, one
 // cooperative launch over the whole device per larger matrix; the
 // rotations applied to Q's rows in parallel in a second launch),
@@ -181,84 +180,6 @@ __device__ __forceinline__ double wsum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
-
-// ---- one wave per matrix, n <= 64 ------------------------------------------
-// For small blocks the barriers of a multi-wave workgroup dominate, so one
-// wave does everything wave-synchronously: lane i owns row i of the matrix
-// in LDS (stride 65: conflict-free across lanes), reductions are shuffles,
-// and the reflector and w go through LDS in program order.  Q is formed
-// with lane c owning column c, so no lane reads another lane's column.
-constexpr int WLD = 65;
-
-__global__ __launch_bounds__(64) void k_syev_wave(const EigTask *__restrict__ tasks, const double *__restrict__ Abase,
-                                                  double *__restrict__ Zbase) {
-  __shared__ double S[64 * WLD], Qs[64 * WLD];
-  __shared__ double v[64], wv[64], tt[64], ee[64];
-  const EigTask tk = tasks[blockIdx.x];
-  const int n = tk.n;
-  if (n <= 0 || n > 64) return;
-  const int lane = threadIdx.x;
-  const bool act = lane < n;
-  const double *__restrict__ A = Abase + tk.a_off;
-  if (act)
-    for (int j = 0; j < n; ++j) S[lane * WLD + j] = A[size_t(j) * n + lane];
-  for (int k = 0; k + 2 < n; ++k) {
-    const bool below = act && lane > k;
-    const double xi = below ? S[lane * WLD + k] : 0.0;
-    const double sigma = wsum((below && lane > k + 1) ? xi * xi : 0.0);
-    const double x0 = __shfl(xi, k + 1, 64);
-    double tau = 0.0, alpha = x0, v0 = x0;
-    if (sigma > 0.0) {
-      const double mu = sqrt(x0 * x0 + sigma);
-      v0 = (x0 <= 0) ? x0 - mu : x0 + mu;
-      alpha = (x0 <= 0) ? mu : -mu;
-      tau = 2.0 / (v0 * v0 + sigma);
-    }
-    const double vi = (below && tau != 0.0) ? ((lane == k + 1) ? v0 : xi) : 0.0;
-    if (lane == 0) {
-      ee[k] = alpha;
-      tt[k] = tau;
-    }
-    if (below) S[lane * WLD + k] = vi;  // the reflector, kept for Q
-    if (tau == 0.0) continue;           // uniform: the trailing block is unchanged
-    v[lane] = vi;
-    double p = 0.0;
-    if (below)
-      for (int j = k + 1; j < n; ++j) p += S[lane * WLD + j] * v[j];
-    p *= tau;
-    const double K = 0.5 * tau * wsum(vi * p);
-    const double wi = p - K * vi;
-    wv[lane] = wi;
-    if (below)
-      for (int j = k + 1; j < n; ++j) S[lane * WLD + j] -= vi * wv[j] + wi * v[j];
-  }
-  // (d, e) to the host
-  double *DE = Zbase + tk.de_off;
-  if (act) {
-    DE[lane] = S[lane * WLD + lane];
-    double el = 0.0;
-    if (lane + 2 < n) el = ee[lane];
-    else if (lane + 2 == n) el = S[(n - 1) * WLD + (n - 2)];
-    DE[n + lane] = el;
-  }
-  // Q = H_0 ... H_{n-3} backwards, lane c owns column c
-  if (act)
-    for (int i = 0; i < n; ++i) Qs[i * WLD + lane] = (i == lane) ? 1.0 : 0.0;
-  for (int k = n - 3; k >= 0; --k) {
-    const double tau = tt[k];
-    if (tau == 0.0) continue;
-    if (act && lane > k) {
-      double sm = 0.0;
-      for (int i = k + 1; i < n; ++i) sm += S[i * WLD + k] * Qs[i * WLD + lane];
-      sm *= tau;
-      for (int i = k + 1; i < n; ++i) Qs[i * WLD + lane] -= S[i * WLD + k] * sm;
-    }
-  }
-  double *__restrict__ Z = Zbase + tk.z_off;
-  if (act)
-    for (int c = 0; c < n; ++c) Z[size_t(c) * n + lane] = Qs[lane * WLD + c];
-}
-
 
 template <int RA, int UP, int UQ, int QG>
 __global__ __launch_bounds__(MT) void k_syev_mid(const EigTask *__restrict__ tasks, double *__restrict__ Abase,
@@ -1123,9 +1044,9 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   // The kernels reduce every matrix to tridiagonal form and build Q; the
   // host runs the QL recurrences; k_syev_apply applies the recorded
   // rotations and writes the sorted eigenpairs.  Reduction size classes:
-  // one wave per matrix up to 64, eight waves with the matrix in LDS up to
-  // 128, one 16-wave workgroup per matrix over global memory up to 256 and
-  // up to 512, else one cooperative launch per matrix over the whole device.
+  // LDS-resident up to 64 and up to 128, one workgroup per matrix over
+  // global memory up to 256 and up to 512, else one cooperative launch per
+  // matrix over the whole device.
   // Workspace: every (d, e) first (one copy to the host), then every Q.
   std::vector<EigTask> t64, t128, t256, t512, tg;
   int64_t desize = 0, qsize = 0;
@@ -1172,7 +1093,7 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   double *Z = reinterpret_cast<double *>(wsb + zoff);
   {
     const EigTask *q = dt;
-    if (!t64.empty()) hipLaunchKernelGGL(k_syev_wave, dim3(unsigned(t64.size())), dim3(64), 0, ctx->stream, q, Abase, Z);
+    if (!t64.empty()) hipLaunchKernelGGL(k_syev_lds<64>, dim3(unsigned(t64.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
     q += t64.size();
     if (!t128.empty())
       hipLaunchKernelGGL(k_syev_lds<128>, dim3(unsigned(t128.size())), dim3(ES), 0, ctx->stream, q, Abase, Z);
