@@ -15,6 +15,7 @@ import os
 import subprocess
 import sys
 import sysconfig
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -47,6 +48,14 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed ({r.returncode}): {cmd[0]}")
 
 
+def _jobs() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
 def _newer(target: str, sources: list[str]) -> bool:
     if not os.path.exists(target):
         return False
@@ -61,8 +70,18 @@ def build_libitx(force: bool = True) -> str:
     deps = srcs + [os.path.join(CSRC, "itx_internal.h"), os.path.join(INCLUDE, "itx.h")]
     if not force and _newer(out, deps):
         return out
+    # one object per source, compiled side by side, then one link
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    hdrs = deps[len(srcs):]
+    todo = [(s, o) for s, o in zip(srcs, objs) if force or not _newer(o, [s, *hdrs])]
+    with ThreadPoolExecutor(max_workers=min(len(todo), _jobs()) or 1) as ex:
+        for f in [ex.submit(_run, [HIPCC, *cflags, "-c", s, "-o", o]) for s, o in todo]:
+            f.result()
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

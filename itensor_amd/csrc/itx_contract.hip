@@ -1341,6 +1341,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
         p->ws = reinterpret_cast<double *>(static_cast<char *>(p->dev) + o_ws);
       }
     }
+    ctx->plans.insert(p);
     *pout = p;
     return ITX_OK;
   };
@@ -1541,7 +1542,10 @@ int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan) {
   if (!plan) return ITX_OK;
   // the device block goes back to the context's stream-ordered pool (no
   // wait); without a context it stays with the pool until itx_destroy
-  if (plan->dev && ctx) itx::pool_free(ctx, plan->dev);
+  if (ctx) {
+    if (plan->dev) itx::pool_free(ctx, plan->dev);
+    ctx->plans.erase(plan);
+  }
   delete plan;
   return ITX_OK;
 }
@@ -1555,6 +1559,7 @@ int itx_chain_create(itx_ctx *ctx, int nstages, itx_plan *const *plans, itx_chai
   ch->plans.assign(plans, plans + nstages);
   ch->args.resize(2 * size_t(nstages));
   ch->nodes.assign(2 * size_t(nstages), nullptr);
+  ctx->chains.insert(ch);
   *out = ch;
   return ITX_OK;
 }
@@ -1619,7 +1624,10 @@ int itx_chain_launch(itx_ctx *ctx, itx_chain *ch, const double *const *A, const 
 
 int itx_chain_destroy(itx_ctx *ctx, itx_chain *ch) {
   if (!ch) return ITX_OK;
-  if (ctx) hipStreamSynchronize(ctx->stream);
+  if (ctx) {
+    hipStreamSynchronize(ctx->stream);
+    ctx->chains.erase(ch);
+  }
   if (ch->exec) hipGraphExecDestroy(ch->exec);
   if (ch->graph) hipGraphDestroy(ch->graph);
   delete ch;

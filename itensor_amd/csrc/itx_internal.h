@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/itx.h"
@@ -110,6 +111,10 @@ struct itx_ctx {
   // tile choice per contraction size class (autotuner results)
   int autotune = 0;
   std::map<std::string, int> tune;
+  // every live plan and chain of the context: itx_destroy releases those
+  // the caller has not (so no HIP object outlives the context)
+  std::unordered_set<itx_plan *> plans;
+  std::unordered_set<itx_chain *> chains;
 };
 
 namespace itx {

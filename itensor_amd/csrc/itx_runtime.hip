@@ -165,8 +165,11 @@ int itx_create(int device, itx_ctx **out) {
 int itx_destroy(itx_ctx *ctx) {
   if (!ctx) return ITX_OK;
   hipStreamSynchronize(ctx->stream);
-  for (auto &kv : ctx->plan_cache) itx_plan_destroy(ctx, kv.second);
+  // chains first (their graphs name the plans' kernels), then every plan
+  // still alive (the one-shot cache's and the caller's), then the pool
+  while (!ctx->chains.empty()) itx_chain_destroy(ctx, *ctx->chains.begin());
   ctx->plan_cache.clear();
+  while (!ctx->plans.empty()) itx_plan_destroy(ctx, *ctx->plans.begin());
   for (auto &kv : ctx->free_blocks) hipFree(kv.second);
   for (auto &kv : ctx->live) hipFree(kv.first);
   hipFree(ctx->red);

@@ -12,6 +12,7 @@ namespace itensor {
 namespace {
 std::mutex g_mu;
 itx_ctx *g_ctx = nullptr;
+bool g_shut = false;
 int g_device = 0;
 int g_prec = ITX_PREC_F64;
 }  // namespace
@@ -55,6 +56,7 @@ void check(int rc, const char *what) {
 itx_ctx *ctx() {
   if (g_ctx) return g_ctx;
   std::lock_guard<std::mutex> lk(g_mu);
+  if (g_shut) throw ITError("itensor: the device context has been shut down");
   if (!g_ctx) {
     itx_ctx *c = nullptr;
     int rc = itx_create(g_device, &c);
@@ -62,6 +64,17 @@ itx_ctx *ctx() {
     g_ctx = c;
   }
   return g_ctx;
+}
+
+itx_ctx *liveCtx() { return g_ctx; }
+
+void shutdown() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_shut = true;
+  if (!g_ctx) return;
+  itx_ctx *c = g_ctx;
+  g_ctx = nullptr;
+  itx_destroy(c);
 }
 
 void setDevice(int device) {

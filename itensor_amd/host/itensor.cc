@@ -574,8 +574,11 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
 }
 
 ContractPlan::~ContractPlan() {
-  if (plan_) itx_plan_destroy(ctx(), plan_);
-  for (auto *p : blockPlans_) itx_plan_destroy(ctx(), p);
+  // after shutdown() the context has released every plan already
+  itx_ctx *c = liveCtx();
+  if (!c) return;
+  if (plan_) itx_plan_destroy(c, plan_);
+  for (auto *p : blockPlans_) itx_plan_destroy(c, p);
 }
 
 std::shared_ptr<Storage> ContractPlan::newOutput() const {

@@ -171,6 +171,17 @@ std::pair<BlockOffsets, int64_t> getBlockOffsets(IndexSet const &is, QN const &d
 // The process-wide libitx context used by the host layer (device 0, the
 // null stream unless setStream is called).
 itx_ctx *ctx();
+// the context if one is open (null before the first use and after
+// shutdown); destructors use it so that they make no HIP call once the
+// context is gone
+itx_ctx *liveCtx();
+// Releases the context and everything it holds (plans, chains, the
+// device pool, pinned buffers) while the HIP runtime is still up.  Device
+// buffers, plans and chains destroyed later are dropped without a HIP
+// call, and ctx() throws from then on.  Registered to run at interpreter
+// exit by the Python module; a C++ program calls it before returning from
+// main.
+void shutdown();
 void setStream(void *stream);
 void setDevice(int device);
 void check(int rc, const char *what);

@@ -133,7 +133,8 @@ struct LocalOp::Program {
   itx_chain *chain = nullptr;
   double flops = 0;
   ~Program() {
-    if (chain) itx_chain_destroy(ctx(), chain);
+    // after shutdown() the context has released the chain already
+    if (chain && liveCtx()) itx_chain_destroy(liveCtx(), chain);
   }
 };
 
@@ -305,18 +306,34 @@ void LocalMPO::position(int b, MPS const &psi) {
 
 Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args_in, LocalMPO const *PH) {
   Args args = args_in;
+  const double noise = args.getReal("Noise", 0.0);
+  const double cutoff = args.getReal("Cutoff", 1e-15);  // MIN_CUT, global.h:61
+  const bool usesvd = args.getBool("UseSVD", false);
   args.add("RespectDegenerate", args.getBool("RespectDegenerate", true));
   Index oldlink = commonIndex(psi(b), psi(b + 1));
   std::string tags = oldlink ? oldlink.tags() : std::string("Link");
   ITensor A = psi(b), B = psi(b + 1);
-  DeltaRhoFn drho;
-  if (PH && args.getReal("Noise", 0.0) > 0)
-    drho = [PH](ITensor const &AA, ITensor const &cmb, Direction d) { return PH->deltaRho(AA, cmb, d); };
-  auto spec = denmatDecomp(phi, A, B, dir, args, drho);
-  if (args.getBool("DoNormalize", false)) {
-    ITensor &oc = (dir == Fromleft ? B : A);
-    const double nrm = norm(oc);
-    if (nrm > 1e-16) oc *= 1.0 / nrm;
+  Spectrum spec;
+  if (usesvd || (noise == 0 && cutoff < 1e-12)) {
+    // the accurate route (mps_impl.h:50-65): A keeps psi(b)'s indices of
+    // phi as U, B becomes V, and D goes into the new centre
+    ITensor D;
+    spec = svd(phi, A, D, B, args);
+    if (args.getBool("DoNormalize", false)) D *= 1.0 / norm(D);
+    if (dir == Fromleft)
+      B *= D;
+    else
+      A *= D;
+  } else {
+    DeltaRhoFn drho;
+    if (PH && noise > 0)
+      drho = [PH](ITensor const &AA, ITensor const &cmb, Direction d) { return PH->deltaRho(AA, cmb, d); };
+    spec = denmatDecomp(phi, A, B, dir, args, drho);
+    if (args.getBool("DoNormalize", false)) {
+      ITensor &oc = (dir == Fromleft ? B : A);
+      const double nrm = norm(oc);
+      if (nrm > 1e-16) oc *= 1.0 / nrm;
+    }
   }
   // put the old tags back onto the new link (mps_impl.h:84-88)
   Index lb = commonIndex(A, B);
@@ -334,6 +351,48 @@ Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const 
   return spec;
 }
 
+Spectrum orthMPS(ITensor &A1, ITensor &A2, Direction dir, Args const &args) {
+  ITensor &L = (dir == Fromleft ? A1 : A2);
+  ITensor &R = (dir == Fromleft ? A2 : A1);
+  Index bnd = commonIndex(L, R);
+  if (!bnd) return Spectrum();
+  ITensor A, B(IndexSet{bnd}, nullptr), D;
+  auto spec = svd(L, A, D, B, args);
+  L = A;
+  R *= (D * B);
+  return spec;
+}
+
+void position(MPS &psi, int i, Args const &args) {
+  const int N = psi.length();
+  if (N == 0) throw ITError("position: MPS is empty");
+  auto linkTags = [&](int b) {
+    Index l = commonIndex(psi(b), psi(b + 1));
+    return l ? l.tags() : std::string("Link");
+  };
+  while (psi.leftLim() < i - 1) {
+    if (psi.leftLim() < 0) psi.leftLim(0);
+    const int b = psi.leftLim() + 1;
+    Args a = args;
+    a.add("LeftTags", linkTags(b));
+    orthMPS(psi.ref(b), psi.ref(b + 1), Fromleft, a);
+    psi.leftLim(psi.leftLim() + 1);
+    if (psi.rightLim() < psi.leftLim() + 2) psi.rightLim(psi.leftLim() + 2);
+  }
+  while (psi.rightLim() > i + 1) {
+    if (psi.rightLim() > N + 1) psi.rightLim(N + 1);
+    const int b = psi.rightLim() - 2;
+    Args a = args;
+    const std::string t = linkTags(b);
+    // LeftTags equal to svd's default RightTags would give U and V the
+    // same tags (mps.cc:720-725)
+    a.add("LeftTags", t == "Link,V" ? std::string("Link") : t);
+    orthMPS(psi.ref(b), psi.ref(b + 1), Fromright, a);
+    psi.rightLim(psi.rightLim() - 1);
+    if (psi.leftLim() > psi.rightLim() - 2) psi.leftLim(psi.rightLim() - 2);
+  }
+}
+
 Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
           std::function<void(BondRecord const &)> obs, std::vector<double> *sweep_seconds) {
   Args args = args_in;
@@ -341,6 +400,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   args.add("DoNormalize", true);
   const int N = psi.length();
   if (int(H.size()) != N) throw ITError("dmrg: MPO and MPS lengths differ");
+  position(psi, 1);  // dmrg.h:373
   LocalMPO PH(H);
   PH.setPrecision(int(args.getInt("HpsiPrecision", ITX_PREC_F64)));
   PH.setFusion(args.getBool("FuseMPO", true));

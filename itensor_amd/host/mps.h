@@ -169,10 +169,22 @@ struct BondRecord {
   double t_env = 0, t_davidson = 0, t_svd = 0;
 };
 
-// MPS::svdBond, density-matrix route (mps/mps_impl.h:21-99)
-// with PH, Args "Noise" > 0 adds the noise term of PH's deltaRho
+// MPS::svdBond (mps/mps_impl.h:21-99).  With Args "UseSVD", or with no
+// noise and Cutoff < 1e-12, the bond is split by svd and the singular
+// values go into the new orthogonality centre; otherwise by the density
+// matrix (denmatDecomp), where with PH and Args "Noise" > 0 the noise term
+// of PH's deltaRho is added.
 Spectrum svdBond(MPS &psi, int b, ITensor const &phi, Direction dir, Args const &args,
                  LocalMPO const *PH = nullptr);
+
+// orthMPS (mps/mps.cc:641-663): A1 (Fromleft) or A2 (Fromright) becomes
+// the U factor of its svd over the shared link, the other tensor absorbs
+// D*V
+Spectrum orthMPS(ITensor &A1, ITensor &A2, Direction dir, Args const &args);
+// MPS::position (mps/mps.cc:667-735, orthMPS route): moves the
+// orthogonality limits so that every site left of i is left-orthogonal and
+// every site right of i right-orthogonal
+void position(MPS &psi, int i, Args const &args = Args());
 
 // dmrg (mps/dmrg.h:51-76, 336-467).  obs (optional) receives one record
 // per bond step; sweep_seconds (optional) the wall time of each sweep.

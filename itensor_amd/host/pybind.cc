@@ -102,6 +102,11 @@ struct PyLocalOp {
 PYBIND11_MODULE(_itensor_host, m) {
   m.doc() = "ITensor-compatible host layer of the MI355X DMRG H*psi engine";
   py::register_exception<ITError>(m, "ITError");
+  // release the device context (plans, graphs, pools) at interpreter exit,
+  // while the HIP runtime is still up; objects freed after that make no
+  // HIP call
+  m.def("shutdown", &itensor::shutdown);
+  py::module_::import("atexit").attr("register")(py::cpp_function([]() { itensor::shutdown(); }));
 
   py::class_<Index>(m, "Index")
       .def(py::init([](uint64_t id, long dim, int dir, int plev, std::string const &tags,
@@ -230,14 +235,29 @@ PYBIND11_MODULE(_itensor_host, m) {
     return out;
   }, py::arg("H"), py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("prec") = 0, py::arg("fuse") = true,
      py::arg("mode") = int(HpsiMode::Graph));
-  m.def("dmrg", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, py::list sweeps, py::dict args) {
-    MPS psi(A);
+  m.def("dmrg", [](std::vector<ITensor> const &H, std::vector<ITensor> const &A, py::list sweeps, py::dict args,
+                   int left_lim, int right_lim) {
+    MPS psi(A, left_lim, right_lim);
     std::vector<py::dict> recs;
     std::vector<double> secs;
     double e = dmrg(psi, H, sweepsFrom(sweeps), argsFrom(args),
                     [&](BondRecord const &r) { recs.push_back(recordDict(r)); }, &secs);
     return py::make_tuple(e, recs, psi.tensors(), secs);
-  });
+  }, py::arg("H"), py::arg("A"), py::arg("sweeps"), py::arg("args"), py::arg("left_lim") = 0,
+     py::arg("right_lim") = 2,
+     "dmrg(H, psi, sweeps, args) (mps/dmrg.h:51-76); left_lim / right_lim are psi's orthogonality limits");
+  m.def("position", [](std::vector<ITensor> const &A, int i, int left_lim, int right_lim, py::dict args) {
+    MPS psi(A, left_lim, right_lim);
+    position(psi, i, argsFrom(args));
+    return py::make_tuple(psi.tensors(), psi.leftLim(), psi.rightLim());
+  }, py::arg("A"), py::arg("i"), py::arg("left_lim"), py::arg("right_lim"), py::arg("args") = py::dict(),
+     "MPS::position(i) (mps/mps.cc:667-735): (tensors, leftLim, rightLim)");
+  m.def("svd_bond", [](std::vector<ITensor> const &A, int b, ITensor const &phi, bool fromleft, py::dict args) {
+    MPS psi(A, fromleft ? b - 1 : b + 1, fromleft ? b + 1 : b + 2);
+    auto spec = svdBond(psi, b, phi, fromleft ? Fromleft : Fromright, argsFrom(args));
+    return py::make_tuple(psi(b), psi(b + 1), spec.eigs, spec.truncerr);
+  }, py::arg("A"), py::arg("b"), py::arg("phi"), py::arg("fromleft"), py::arg("args") = py::dict(),
+     "MPS::svdBond(b, phi, dir, args) (mps/mps_impl.h:21-99): (A_b, A_b+1, eigs, truncerr)");
   py::class_<PyLocalOp>(m, "LocalOp")
       .def(py::init<ITensor, ITensor, ITensor, ITensor, bool>(), py::arg("W1"), py::arg("W2"), py::arg("L"),
            py::arg("R"), py::arg("fuse") = true,

@@ -10,9 +10,11 @@ raises ImportError -- there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -111,6 +113,18 @@ class ItxError(RuntimeError):
     pass
 
 
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all_contexts():
+    """Destroy every open context at interpreter exit, while the HIP runtime
+    is still up (itx_destroy also releases the plans and chains a context
+    still holds); Plan / Chain objects freed later make no library call."""
+    for c in list(_live_contexts):
+        c.close()
+
+
 class Context:
     """An itx_ctx bound to one device and one HIP stream."""
 
@@ -121,6 +135,7 @@ class Context:
         if rc != ITX_OK:
             raise ItxError(f"itx_create failed ({rc}): {self.L.itx_last_error(None).decode()}")
         self.h = h
+        _live_contexts.add(self)
         if stream is not None:
             self.set_stream(stream)
 
@@ -299,9 +314,10 @@ class Plan:
                                                    float(alpha), float(beta)), "itx_plan_execute")
 
     def close(self):
-        if self.h:
+        # a destroyed context has released its plans already
+        if self.h and self.ctx.h:
             self.ctx.L.itx_plan_destroy(self.ctx.h, self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -329,9 +345,10 @@ class Chain:
             None if be is None else be.ctypes.data_as(_dp)), "itx_chain_launch")
 
     def close(self):
-        if self.h:
+        # a destroyed context has released its chains already
+        if self.h and self.ctx.h:
             self.ctx.L.itx_chain_destroy(self.ctx.h, self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

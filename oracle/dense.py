@@ -8,8 +8,11 @@ result order is the given C labels (default: uncontracted A then B, as
 contractIS, indexset_impl.h:79-127).  The permute->GEMM->permute plan of
 CProps is the same arithmetic as numpy.tensordot + transpose.
 
-Pinned by: brute-force loops over the reference's contract_test cases
-(tests/test_oracle_dense.py).
+Pinned by tests/test_oracle_dense.py: contract() against explicit loops
+over the reference's own cases (unittest/contract_test.cc:161-486, Case 1-4,
+NM1-NM5, M1-M4, Regression Test 1); hpsi_dense against a term-by-term sum
+of the effective Hamiltonian; each HPSI_STAGES entry against the index
+order and values of the ITensor-level chain (oracle.dmrg.local_op_product).
 """
 from __future__ import annotations
 

@@ -7,7 +7,7 @@ Restates:
   truncate ...................... decomp.cc:305-463
   diag_hermitian (dense, QN) .... hermitian.cc:40-426, tensor/algs_impl.h:91-139
   denmatDecomp .................. decomp.h:316-418
-  MPS::svdBond (denmat route) ... mps/mps_impl.h:21-99
+  MPS::svdBond (both routes) .... mps/mps_impl.h:21-99
   LocalOp::product .............. mps/localop.h:324-365
   LocalMPO position/makeL/makeR . mps/localmpo.h:443-618
   DMRGWorker / sweepnext ........ mps/dmrg.h:336-467, mps/sweeps.h:489-500
@@ -595,6 +595,41 @@ def denmat_decomp(AA: ITensor, A: ITensor, B: ITensor, fromleft: bool, args: dic
     return newoc_new, to_orth_new, spec
 
 
+def svd_bond(A: ITensor, B: ITensor, phi: ITensor, fromleft: bool, args: dict):
+    """MPS::svdBond (mps/mps_impl.h:21-99) for the pair (A, B) = (psi(b),
+    psi(b+1)), noise 0.  With UseSVD, or Cutoff < 1e-12 and no noise
+    (:50), svd(phi, A, D, B) with D (normalised under DoNormalize) pushed
+    into the new centre (:52-65); otherwise denmatDecomp with the centre
+    renormalised (:66-80).  Returns (A_new, B_new, spec); spec["route"] is
+    "svd" or "denmat"."""
+    noise = args.get("Noise", 0.0)
+    cutoff = args.get("Cutoff", MIN_CUT)
+    if args.get("UseSVD", False) or (noise == 0 and cutoff < 1e-12):
+        if noise > 0:
+            raise ValueError("svd: noise term not implemented (decomp.cc:183-184)")
+        U, D, V, spec = svd(phi, A.inds, args)
+        if args.get("DoNormalize", False):
+            D = D * (1.0 / norm(D))
+        if fromleft:
+            V = V * D
+        else:
+            U = U * D
+        spec = dict(spec, route="svd")
+        return U, V, spec
+    Anew, Bnew, spec = denmat_decomp(phi, A, B, fromleft, args)
+    if args.get("DoNormalize", False):
+        oc = Bnew if fromleft else Anew
+        nrm = norm(oc)
+        if nrm > 1e-16:
+            oc = oc * (1.0 / nrm)
+        if fromleft:
+            Bnew = oc
+        else:
+            Anew = oc
+    spec = dict(spec, route="denmat")
+    return Anew, Bnew, spec
+
+
 # ---------------------------------------------------------- davidson ----
 def davidson(product, phi: ITensor, args: dict, size: int | None = None, rng_seed: int = 12345):
     """davidson (iterativesolvers.h:99-451) for one eigenpair.
@@ -759,27 +794,23 @@ def dmrg(H, psi: MPS, sweeps, quiet=True, record=None):
     energy = float("nan")
     for sw, s in enumerate(sweeps, start=1):
         args = {"Cutoff": s.get("Cutoff", 1e-8), "MinDim": s.get("MinDim", 1), "MaxDim": s["MaxDim"],
-                "MaxIter": s.get("Niter", 2), "RespectDegenerate": True, "Tags": "Link"}
+                "MaxIter": s.get("Niter", 2), "RespectDegenerate": True, "Tags": "Link",
+                "DoNormalize": True, "Noise": s.get("Noise", 0.0)}
+        if args["Noise"] > 0:
+            raise NotImplementedError("oracle dmrg: the noise term is not restated")
         b, ha = 1, 1
         while ha <= 2:
             PH.position(b, psi)
             phi = psi(b) * psi(b + 1)
             energy, phi, nprod = davidson(PH.product, phi, args)
             fromleft = ha == 1
-            Anew, Bnew, spec = denmat_decomp(phi, psi(b), psi(b + 1), fromleft, args)
-            oc = Bnew if fromleft else Anew
-            nrm = norm(oc)
-            if nrm > 1e-16:
-                oc = oc * (1.0 / nrm)
-            if fromleft:
-                Bnew = oc
-            else:
-                Anew = oc
+            Anew, Bnew, spec = svd_bond(psi(b), psi(b + 1), phi, fromleft, args)
             psi.set(b, Anew)
             psi.set(b + 1, Bnew)
             if record is not None:
                 record({"sweep": sw, "b": b, "ha": ha, "energy": energy, "truncerr": spec["truncerr"],
-                        "dim": spec["newmid"].dim, "nprod": nprod})
+                        "dim": common_index(Anew, Bnew).dim, "nprod": nprod,
+                        "route": spec["route"]})
             b, ha = sweepnext(b, ha, N)
     return energy, psi
 

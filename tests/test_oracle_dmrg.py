@@ -243,3 +243,85 @@ def test_oracle_svd_reconstruction_and_factor():
     a, b = Index(6, "a"), Index(9, "b")
     U, D, V, spec = OD.svd(dense([a, b], M), [a], {})
     assert np.allclose(np.sqrt(spec["eigs"]), np.linalg.svd(M, compute_uv=False), atol=1e-13)
+
+
+# ---- MPS::svdBond routes (mps/mps_impl.h:21-99) ------------------------
+def _bond_problem(qn: bool, seed: int = 5):
+    N = 6
+    sites = OD.spin_half_sites(N, qn)
+    psi = OD.random_mps(sites, 4, seed=seed) if not qn else OD.product_mps(sites, [i % 2 for i in range(N)])
+    if qn:
+        # grow the links of the product state with two DMRG sweeps
+        Hm = heisenberg_mpo(sites)
+        _, psi = dmrg(Hm, psi, [{"MaxDim": 8, "Cutoff": 1e-10}, {"MaxDim": 8, "Cutoff": 1e-10}])
+    b = 3
+    return psi(b), psi(b + 1), psi(b) * psi(b + 1)
+
+
+@pytest.mark.parametrize("qn", [False, True])
+@pytest.mark.parametrize("args,route", [
+    ({"Cutoff": 1e-13, "MaxDim": 100}, "svd"),
+    ({"Cutoff": 1e-10, "MaxDim": 100}, "denmat"),
+    ({"Cutoff": 1e-10, "MaxDim": 100, "UseSVD": True}, "svd"),
+    ({"MaxDim": 100}, "svd"),  # Cutoff defaults to MIN_CUT
+])
+def test_svd_bond_route_and_identities(qn, args, route):
+    """The route follows mps_impl.h:50 (UseSVD, or noise 0 and cutoff below
+    1e-12).  Both routes reconstruct phi exactly when nothing is truncated,
+    leave the side being orthogonalised an isometry, and (DoNormalize) give a
+    unit-norm centre."""
+    A, B, phi = _bond_problem(qn)
+    a = dict(args, RespectDegenerate=True, DoNormalize=True)
+    for fromleft in (True, False):
+        An, Bn, spec = OD.svd_bond(A, B, phi, fromleft, a)
+        assert spec["route"] == route
+        AB = An * Bn
+        # phi has unit norm up to the ordering of the product: compare the
+        # normalised product with the normalised phi
+        from oracle.itensor import add_into
+        diff = add_into(AB * (1.0 / norm(AB)), phi * (1.0 / norm(phi)), -1.0)
+        assert norm(diff) < 1e-12
+        ortho, centre = (An, Bn) if fromleft else (Bn, An)
+        assert abs(norm(centre) - 1.0) < 1e-13
+        link = OD.common_index(An, Bn)
+        G = ortho * ortho.prime(1, link).dag()
+        from oracle.itensor import remove_qns
+        g = remove_qns(G).d if G.is_qn else G.d
+        assert np.allclose(g, np.eye(link.dim), atol=1e-12)
+
+
+def test_svd_bond_route_keeps_small_singular_values():
+    """Cutoff below 1e-12 with no noise takes the svd route, which resolves
+    singular values below sqrt(machine epsilon): a matrix with spectrum
+    10^-k (k = 0..9) keeps all ten, and the truncation error of dropping the
+    smallest at MaxDim 9 is 1e-18 relative (the density-matrix eigenvalues
+    of those values sit at the rounding level of the largest one)."""
+    i, j = Index(10, "i"), Index(12, "j")
+    rng = np.random.default_rng(0)
+    Q1, _ = np.linalg.qr(rng.standard_normal((10, 10)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((12, 10)))
+    s = 10.0 ** -np.arange(10)
+    M = (Q1 * s) @ Q2.T
+    phi = dense([i, j], M)
+    l = Index(1, "Link,l=1")
+    A = dense([i, l], np.ones((10, 1)))
+    B = dense([l, j], np.ones((1, 12)))
+    An, Bn, spec = OD.svd_bond(A, B, phi, True, {"Cutoff": 1e-17, "MaxDim": 9, "RespectDegenerate": True})
+    assert spec["route"] == "svd"
+    assert OD.common_index(An, Bn).dim == 9
+    assert abs(spec["truncerr"] - 1e-18 / np.sum(s ** 2)) < 1e-24
+    assert np.allclose(np.sqrt(spec["eigs"]), s[:9], rtol=1e-8, atol=0)
+
+
+def test_qn_dmrg_svd_route_vs_ed():
+    """N=10 QN Heisenberg at cutoff 1e-13 and no noise: every bond step takes
+    the svd route and the energy reaches exact diagonalisation."""
+    N = 10
+    sites = OD.spin_half_sites(N, True)
+    Hm = heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)])
+    sw = [{"MaxDim": m, "Cutoff": 1e-13, "Niter": 2} for m in (10, 20, 40, 40, 40)]
+    trace = []
+    E, _ = dmrg(Hm, psi0, sw, record=trace.append)
+    assert {t["route"] for t in trace} == {"svd"}
+    assert abs(E - GOLD["energies"]["10"]) < 1e-10
