@@ -43,58 +43,9 @@ METRIC = "DMRG sec/sweep + Hψ TFLOP/s, mixed vs FP64 on 1 MI355X at |ΔE|≤1e-
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
 SEED = 20261202 + 2
 
-
-def stage_dims(chi: int, d: int = 2, k: int = 5):
-    psi = (chi, d, d, chi)
-    Ld = (chi, k, chi)
-    W = (k, d, d, k)
-    T1 = (d, d, chi, k, chi)
-    T2 = (d, chi, chi, d, k)
-    T3 = (chi, chi, d, d, k)
-    return [(psi, Ld, T1), (T1, W, T2), (T2, W, T3), (T3, Ld, psi)]
-
-
-# labels of the four stages as ITensor's computeLabels assigns them
-STAGES = [
-    ((-1, 2, 3, 4), (-1, 5, 6), (2, 3, 4, 5, 6)),
-    ((-1, 3, 4, -2, 5), (-2, -1, 6, 7), (3, 4, 5, 6, 7)),
-    ((-1, 3, 4, 5, -2), (-2, -1, 6, 7), (3, 4, 5, 6, 7)),
-    ((-1, 3, 4, 5, -2), (-1, -2, 6), (3, 4, 5, 6)),
-]
-# fused chain of LocalOp::product: S1, then T1 x W12 with W12 = W1*W2
-# (h0,s1,s1',s2,s2',h2), then S4 -- same result layout and flop count
-FUSED = [
-    STAGES[0],
-    ((-1, -2, 3, -3, 4), (-3, -1, 5, -2, 6, 7), (3, 4, 5, 6, 7)),
-    STAGES[3],
-]
-
-
-def fused_dims(chi: int, d: int = 2, k: int = 5):
-    psi = (chi, d, d, chi)
-    Ld = (chi, k, chi)
-    T1 = (d, d, chi, k, chi)
-    W12 = (k, d, d, d, d, k)
-    T3 = (chi, chi, d, d, k)
-    return [(psi, Ld, T1), (T1, W12, T3), (T3, Ld, psi)]
-
-
-def heisenberg_w() -> np.ndarray:
-    """Bulk Heisenberg MPO tensor W(h_in, s, s', h_out) (J = Jz = 1)."""
-    Id = np.eye(2)
-    Sz = np.diag([0.5, -0.5])
-    Sp = np.zeros((2, 2)); Sp[1, 0] = 1.0
-    Sm = np.zeros((2, 2)); Sm[0, 1] = 1.0
-    W = np.zeros((5, 2, 2, 5))
-    W[0, :, :, 0] = Id; W[1, :, :, 1] = Id
-    W[2, :, :, 0] = Sz; W[1, :, :, 2] = Sz
-    W[3, :, :, 0] = Sm; W[1, :, :, 3] = 0.5 * Sp
-    W[4, :, :, 0] = Sp; W[1, :, :, 4] = 0.5 * Sm
-    return W
-
-
-def hpsi_flops(chi: int, d: int = 2, k: int = 5) -> float:
-    return 4.0 * d * d * k * chi ** 3 + 4.0 * d ** 3 * k * k * chi ** 2
+from itensor_amd.workloads import (DMRG_MAXDIM, FUSED, HUBBARD_MAXDIM, SECTORS_1024, STAGES,  # noqa: E402
+                                   fused_dims, heisenberg_qn_chain, heisenberg_w, hpsi_flops, hubbard_cylinder,
+                                   qn_operands, stage_dims)
 
 
 def cpu_baseline(psi, L, W, R, budget_s: float = 20.0):
@@ -102,12 +53,7 @@ def cpu_baseline(psi, L, W, R, budget_s: float = 20.0):
     cores: 1 warm-up, then repeats until ~budget_s, median reported."""
     from oracle.dense import hpsi_dense
 
-    try:
-        from threadpoolctl import threadpool_info
-
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:
-        threads = os.cpu_count() or 1
+    threads, _, _ = host_cpu()
     t0 = time.perf_counter()
     ref = hpsi_dense(psi, L, W, W, R)
     first = time.perf_counter() - t0
@@ -118,40 +64,6 @@ def cpu_baseline(psi, L, W, R, budget_s: float = 20.0):
         hpsi_dense(psi, L, W, W, R)
         times.append(time.perf_counter() - t0)
     return ref, float(np.median(times)), threads, reps
-
-
-# bond sectors q = 2 Sz in {-30..30} of the chi = 1024 Sz-conserving
-# Heisenberg bond at the chain midpoint (binomial weights, max 120, min 1)
-SECTORS_1024 = [1, 1, 1, 1, 1, 1, 3, 7, 14, 25, 41, 60, 81, 101, 114, 120, 114, 101, 81, 60, 41, 25, 14, 7, 3,
-                1, 1, 1, 1, 1, 1]
-
-
-def qn_operands(H, sectors, seed):
-    """Block-sparse two-site operands: phi(dag(l), s1, s2, r), L(l, hl,
-    dag(l)'), R(dag(r), dag(hr), r'), W(dag(s), s', dag(h_in), h_out);
-    site {+1: 1, -1: 1}, MPO link {0: 3, -2: 1, +2: 1}; flux-0 blocks of
-    phi, L, R filled with N(0,1)."""
-    def sz(v):
-        return [("Sz", int(v))]
-
-    ids = iter(range(7001, 7100))
-
-    def idx(qns, tags):
-        return H.Index(next(ids), sum(d for _, d in qns), 1, 0, tags, qns)
-
-    bond = [(sz(q), d) for q, d in zip(range(-30, 31, 2), sectors)]
-    l, r = idx(bond, "Link,l=49"), idx(bond, "Link,l=51")
-    site = [(sz(1), 1), (sz(-1), 1)]
-    s1, s2 = idx(site, "Site,n=50"), idx(site, "Site,n=51")
-    hq = [(sz(0), 3), (sz(-2), 1), (sz(2), 1)]
-    hl, hm, hr = idx(hq, "Link,h=49"), idx(hq, "Link,h=50"), idx(hq, "Link,h=51")
-    W = heisenberg_w().transpose(1, 2, 0, 3)  # (s, s', h_in, h_out)
-    W1 = H.from_dense([s1.dag(), s1.prime(), hl.dag(), hm], W)
-    W2 = H.from_dense([s2.dag(), s2.prime(), hm.dag(), hr], W)
-    phi = H.random_itensor([l.dag(), s1, s2, r], [], seed)
-    L = H.random_itensor([l, hl, l.dag().prime()], [], seed + 1)
-    R = H.random_itensor([r.dag(), hr.dag(), r.prime()], [], seed + 2)
-    return phi, L, W1, W2, R
 
 
 def qn_hpsi(steps: int, warmup: int):
@@ -201,131 +113,58 @@ def qn_hpsi(steps: int, warmup: int):
     return res
 
 
-def heisenberg_qn_chain(H, N: int):
-    """Sz-conserving spin-1/2 Heisenberg chain (J = Jz = 1, open ends):
-    site indices, the MPO (bulk W(dag(s), s', dag(h_in), h_out), first and
-    last site contracted with the start / end states of the MPO link,
-    sample/Heisenberg.h:76-116) and the Neel product state with link QNs
-    that give every site tensor flux 0."""
-    def sz(v):
-        return [("Sz", int(v))]
-
-    ids = iter(range(100001, 100001 + 4 * N + 8))
-
-    def idx(qns, tags):
-        return H.Index(next(ids), sum(d for _, d in qns), 1, 0, tags, qns)
-
-    sites = [idx([(sz(1), 1), (sz(-1), 1)], f"Site,S=1/2,n={n}") for n in range(1, N + 1)]
-    hq = [(sz(0), 3), (sz(-2), 1), (sz(2), 1)]
-    hl = [idx(hq, f"Link,H,l={n}") for n in range(N + 1)]
-    Wb = heisenberg_w().transpose(1, 2, 0, 3)  # (s, s', h_in, h_out)
-    mpo = []
-    for n in range(1, N + 1):
-        s, row, col = sites[n - 1], hl[n - 1].dag(), hl[n]
-        if n == 1:
-            W, inds = Wb[:, :, 1, :], [s.dag(), s.prime(), col]
-        elif n == N:
-            W, inds = Wb[:, :, :, 0], [s.dag(), s.prime(), row]
-        else:
-            W, inds = Wb, [s.dag(), s.prime(), row, col]
-        mpo.append(H.from_dense(inds, np.ascontiguousarray(W)))
-    # Neel state: link l_n carries -(sum of the site charges 1..n)
-    A, q, prev = [], 0, None
-    for n in range(1, N + 1):
-        st = (n - 1) % 2  # 0 = Up (Sz +1), 1 = Dn
-        q -= 1 if st == 0 else -1
-        s = sites[n - 1]
-        v = np.zeros(2)
-        v[st] = 1.0
-        if n < N:
-            link = idx([(sz(q), 1)], f"Link,l={n}")
-        if n == 1:
-            A.append(H.from_dense([s, link], v.reshape(2, 1)))
-        elif n == N:
-            A.append(H.from_dense([prev.dag(), s], v.reshape(1, 2)))
-        else:
-            A.append(H.from_dense([prev.dag(), s, link], v.reshape(1, 2, 1)))
-        prev = link if n < N else None
-    return mpo, A
 
 
-def hubbard_cylinder(H, Nx: int, Ny: int, t: float = 1.0, U: float = 4.0, yperiodic: bool = True):
-    """Hubbard model on the Nx x Ny square lattice (YPeriodic cylinder, site
-    order of squareLattice) with (Nf, Sz) conserved: the MPO from the host
-    layer's AutoMPO (-t c+_i,s c_j,s + h.c. on every bond, U n_up n_dn on
-    every site) and a half-filled product state (Up, Dn, Up, ...) whose link
-    QNs give every site tensor but the last flux 0; the last one carries the
-    state's total charge (Nf = N, Sz = 0)."""
-    N = Nx * Ny
-    sites = H.electron_sites(N)
-    ampo = H.AutoMPO(sites)
-    for (i, j) in H.square_lattice(Nx, Ny, yperiodic):
-        for c, a in (("Cdagup", "Cup"), ("Cdagdn", "Cdn")):
-            ampo.add(-t, c, i, a, j)
-            ampo.add(-t, c, j, a, i)
-    for i in range(1, N + 1):
-        ampo.add(U, "Nupdn", i)
-    mpo = ampo.to_mpo()
-    ids = iter(range(300001, 300001 + N + 8))
-    A, nf, sz, prev = [], 0, 0, None
-    for n in range(1, N + 1):
-        st = 1 if (n - 1) % 2 == 0 else 2  # Up, Dn
-        nf += 1
-        sz += 1 if st == 1 else -1
-        s = sites(n)
-        v = np.zeros(4)
-        v[st] = 1.0
-        if n < N:
-            q = [(k, x) for k, x in (("Nf", -nf), ("Sz", -sz)) if x]
-            link = H.Index(next(ids), 1, 1, 0, f"Link,l={n}", [(q, 1)])
-        if n == 1:
-            A.append(H.from_dense([s, link], v.reshape(4, 1)))
-        elif n == N:
-            tot = [(k, x) for k, x in (("Nf", nf), ("Sz", sz)) if x]
-            A.append(H.from_dense([prev.dag(), s], v.reshape(1, 4), tot))
-        else:
-            A.append(H.from_dense([prev.dag(), s, link], v.reshape(1, 4, 1)))
-        prev = link if n < N else None
-    return mpo, A
+def _leg_summary(E, recs, secs, wall, nsweeps):
+    last = [r for r in recs if r["sweep"] == nsweeps]
+    fl = np.sum([r["hpsi_flops"] for r in recs], axis=0)
+    return {"energy": E, "wall_s": round(wall, 3), "sec_per_sweep": [round(x, 4) for x in secs],
+            "max_dim": [max(r["dim"] for r in recs if r["sweep"] == k + 1) for k in range(nsweeps)],
+            "truncerr_max_last_sweep": max(r["truncerr"] for r in last),
+            "hpsi_products_last_sweep": sum(r["nprod"] for r in last),
+            # share of the H*psi flops whose kernels computed in each form
+            "hpsi_flops_by_form": {k: round(float(v / max(fl.sum(), 1.0)), 4)
+                                   for k, v in zip(("f64", "f32", "bf16x3"), fl)},
+            "hpsi_flops_total": float(fl.sum())}
 
 
-DMRG_MAXDIM = [10, 20, 50, 100, 200, 400, 800, 1200, 1600, 2048]
+def _phase_split(recs, nsweeps):
+    last = [r for r in recs if r["sweep"] == nsweeps]
+    return {"env": round(sum(r["t_env"] for r in last), 4), "davidson": round(sum(r["t_davidson"] for r in last), 4),
+            "svd": round(sum(r["t_svd"] for r in last), 4)}
 
 
-def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0):
+def dmrg_leg(N: int, maxdims, prec: int, mode: int, final_fp64: int = 0, mindim_follows: bool = False,
+             sync_timing: bool = False):
     """QN Heisenberg DMRG (N sites, Neel start, one sweep per maxdim entry,
     cutoff 1e-10, Davidson niter 2) in one H*psi form: wall time of each
-    sweep (everything included) and how it splits over the phases of the
-    bond steps (environment update, davidson, svdBond)."""
+    sweep (everything included, no added synchronisation).  With
+    mindim_follows, MinDim = MaxDim so that the bond dimension follows the
+    ramp instead of saturating at the cutoff.  With sync_timing, the device
+    is synchronised between the phases of every bond step and the phase
+    split of the last sweep is reported (a diagnostic run, slower)."""
     import torch
 
     from itensor_amd import host as H
 
     mpo, A = heisenberg_qn_chain(H, N)
-    sweeps = [{"MaxDim": int(m), "Cutoff": 1e-10, "Niter": 2} for m in maxdims]
+    sweeps = [{"MaxDim": int(m), "MinDim": int(m) if mindim_follows else 1, "Cutoff": 1e-10, "Niter": 2}
+              for m in maxdims]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "HpsiMode": mode, "SyncTiming": True,
+    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "HpsiMode": mode, "SyncTiming": sync_timing,
                                                "FinalFP64Sweeps": final_fp64})
     wall = time.perf_counter() - t0
-    last = [r for r in recs if r["sweep"] == len(maxdims)]
-    return {"energy": E, "wall_s": round(wall, 3), "sec_per_sweep": [round(x, 4) for x in secs],
-            "max_dim": [max(r["dim"] for r in recs if r["sweep"] == k + 1) for k in range(len(maxdims))],
-            "truncerr_max_last_sweep": max(r["truncerr"] for r in last),
-            "last_sweep_split_s": {"env": round(sum(r["t_env"] for r in last), 4),
-                                   "davidson": round(sum(r["t_davidson"] for r in last), 4),
-                                   "svd": round(sum(r["t_svd"] for r in last), 4)},
-            "hpsi_products_last_sweep": sum(r["nprod"] for r in last)}
+    out = _leg_summary(E, recs, secs, wall, len(maxdims))
+    if sync_timing:
+        out["last_sweep_split_s"] = _phase_split(recs, len(maxdims))
+    return out
 
 
-HUBBARD_MAXDIM = [50, 100, 200, 400, 400, 400, 400, 400]  # ramp, then converged at 400
-
-
-def hubbard_leg(Nx: int, Ny: int, maxdims, prec: int, final_fp64: int = 0):
+def hubbard_leg(Nx: int, Ny: int, maxdims, prec: int, final_fp64: int = 0, sync_timing: bool = False):
     """(Nf, Sz)-conserving Hubbard DMRG on the Nx x Ny YPeriodic cylinder
     (t = 1, U = 4, half filling, cutoff 1e-9, niter 2, noise 1e-6 / 1e-7 on
-    the first two sweeps) in one H*psi form: wall time per sweep and the
-    phase split of the last sweep."""
+    the first two sweeps) in one H*psi form: wall time per sweep."""
     import torch
 
     from itensor_amd import host as H
@@ -335,16 +174,74 @@ def hubbard_leg(Nx: int, Ny: int, maxdims, prec: int, final_fp64: int = 0):
     sweeps = [{"MaxDim": int(m), "Cutoff": 1e-9, "Niter": 2, "Noise": z} for m, z in zip(maxdims, noise)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "SyncTiming": True,
+    E, recs, _, secs = H.dmrg(mpo, A, sweeps, {"HpsiPrecision": prec, "SyncTiming": sync_timing,
                                                "FinalFP64Sweeps": final_fp64})
     wall = time.perf_counter() - t0
-    last = [r for r in recs if r["sweep"] == len(maxdims)]
-    return {"energy": E, "wall_s": round(wall, 3), "sec_per_sweep": [round(x, 4) for x in secs],
-            "max_dim": [max(r["dim"] for r in recs if r["sweep"] == k + 1) for k in range(len(maxdims))],
-            "truncerr_max_last_sweep": max(r["truncerr"] for r in last),
-            "last_sweep_split_s": {"env": round(sum(r["t_env"] for r in last), 4),
-                                   "davidson": round(sum(r["t_davidson"] for r in last), 4),
-                                   "svd": round(sum(r["t_svd"] for r in last), 4)}}
+    out = _leg_summary(E, recs, secs, wall, len(maxdims))
+    if sync_timing:
+        out["last_sweep_split_s"] = _phase_split(recs, len(maxdims))
+    return out
+
+
+def host_cpu():
+    """(BLAS threads the host oracle runs with, cores this process may use,
+    CPU model).  The BLAS thread count is what the numpy legs use; the
+    oracle's Python loops around it run on one core."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        nproc = os.cpu_count() or 1
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # pragma: no cover
+        threads = nproc
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, nproc, model
+
+
+def cpu_qn_hpsi(budget_s: float = 10.0):
+    """The oracle's block-sparse H*psi (per-block numpy loop, the
+    reference's doTask loop restated) on the chi = 1024 / 31-sector operands
+    on the host cores: median of repeats within budget_s."""
+    from oracle.dmrg import local_op_product
+    from tests.hpsi_cases import hpsi_operands
+
+    phi, L, W1, W2, R = hpsi_operands(SECTORS_1024, seed=20261202 + 3)
+    t0 = time.perf_counter()
+    local_op_product(phi, L, W1, W2, R)
+    first = time.perf_counter() - t0
+    times = []
+    for _ in range(max(1, min(20, int(budget_s / max(first, 1e-3))))):
+        t0 = time.perf_counter()
+        local_op_product(phi, L, W1, W2, R)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), len(times)
+
+
+def cpu_dmrg(N: int, maxdims):
+    """The oracle's QN DMRG (numpy restatement of DMRGWorker, davidson and
+    denmatDecomp) on the host cores for the first sweeps of the ramp: wall
+    time per sweep."""
+    from oracle import dmrg as OD
+
+    sites = OD.spin_half_sites(N, True)
+    Hm = OD.heisenberg_mpo(sites)
+    psi = OD.product_mps(sites, [i % 2 for i in range(N)])
+    secs, E = [], None
+    for m in maxdims:
+        t0 = time.perf_counter()
+        E, psi = OD.dmrg(Hm, psi, [{"MaxDim": m, "Cutoff": 1e-10, "Niter": 2}])
+        secs.append(time.perf_counter() - t0)
+    return secs, E
 
 
 def measured_traffic(chi: int, prec: str, stage: str):
@@ -398,6 +295,12 @@ def main():
                     help="Hubbard cylinder leg: NxxNy:maxdim1,maxdim2,... ('' skips it)")
     ap.add_argument("--hubbard-forms", default="f64,f32,bf16x3+f64last",
                     help="H*psi forms of the Hubbard leg ('+f64last': the last sweep in FP64)")
+    ap.add_argument("--dmrg-big", default="100:full",
+                    help="QN DMRG leg with MinDim = MaxDim (bond dimension follows the ramp to 2048); '' skips it")
+    ap.add_argument("--dmrg-big-forms", default="f64,f32,bf16x3",
+                    help="H*psi forms of the MinDim = MaxDim DMRG leg")
+    ap.add_argument("--phase-split", action="store_true",
+                    help="also run each DMRG workload once in FP64 with a synchronised phase split")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="calibration only: time a plain device copy of this many GiB (for PMC byte counters)")
     args = ap.parse_args()
@@ -520,52 +423,95 @@ def main():
         line["forms"][prec] = {"tflops": round(flops / (r["wall_s"] / args.steps) / 1e12, 3),
                                "ms_per_hpsi": round(r["wall_s"] / args.steps * 1e3, 4)}
 
+    cores, nproc, cpu_model = host_cpu()
+    cpu_model = "%s (%d cores visible)" % (cpu_model, nproc)
     if not args.no_qn:
         q = qn_hpsi(max(5, args.steps), args.warmup)
         line["qn_hpsi"] = {"workload": "block-sparse two-site Hpsi, chi=1024 over 31 Sz sectors "
                                        "(max 120, min 1), site {+1,-1}, MPO link {0:3,-2:1,+2:1}", **q}
+        if not args.no_cpu:
+            t_q, reps = cpu_qn_hpsi()
+            qf = q["graph"]["flops"]
+            line["qn_hpsi"]["cpu_baseline"] = {
+                "value": round(qf / t_q / 1e12, 6), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+                "cpu": cpu_model,
+                "sample": "oracle local_op_product (per-block numpy loop) on the same structure, median of %d: "
+                          "%.4f s per H*psi" % (reps, t_q)}
+
+    def mixed_deltas(res, forms):
+        if "f64" in res:
+            for form in forms:
+                if form != "f64" and form in res:
+                    res["dE_%s_vs_f64" % form] = res[form]["energy"] - res["f64"]["energy"]
+                    res["sweep_speedup_%s_vs_f64" % form] = round(res["f64"]["wall_s"] / res[form]["wall_s"], 3)
 
     if args.dmrg:
         n_s, md_s = args.dmrg.split(":")
         mds = DMRG_MAXDIM if md_s == "full" else [int(x) for x in md_s.split(",")]
+        forms = args.dmrg_forms.split(",")
         dm = {"workload": "QN Heisenberg N=%s open chain (total Sz 0, Neel start), maxdim %s, cutoff 1e-10, "
-                          "niter 2; H*psi in the named form, everything else FP64" % (n_s, mds)}
-        for form in args.dmrg_forms.split(","):
+                          "niter 2; H*psi in the named form, everything else FP64; sec/sweep without added "
+                          "synchronisation" % (n_s, mds)}
+        for form in forms:
             base, _, last = form.partition("+")
             dm[form] = dmrg_leg(int(n_s), mds, {"f64": 0, "f32": 1, "bf16x3": 2}[base], 2,
                                 1 if last == "f64last" else 0)
             print("dmrg leg %s done in %.1f s" % (form, dm[form]["wall_s"]), file=sys.stderr, flush=True)
-        if "f64" in dm:
-            for form in args.dmrg_forms.split(","):
-                if form != "f64":
-                    dm["dE_%s_vs_f64" % form] = dm[form]["energy"] - dm["f64"]["energy"]
-                    dm["sweep_speedup_%s_vs_f64" % form] = round(dm["f64"]["wall_s"] / dm[form]["wall_s"], 3)
+        mixed_deltas(dm, forms)
+        dm["phase_split_f64"] = dmrg_leg(int(n_s), mds, 0, 2, sync_timing=True)["last_sweep_split_s"]
+        if not args.no_cpu:
+            k = 3
+            secs, e_cpu = cpu_dmrg(int(n_s), mds[:k])
+            dm["cpu_baseline"] = {
+                "value": round(secs[-1], 3), "unit": "s/sweep (maxdim %d)" % mds[k - 1], "cores": cores,
+                "kind": "port", "cpu": cpu_model,
+                "sample": "oracle dmrg (numpy restatement of DMRGWorker, davidson, denmatDecomp) on the same "
+                          "chain, sweeps 1-%d of the ramp (maxdim %s): %s s per sweep; the GPU FP64 run's same "
+                          "sweeps: %s s" % (k, mds[:k], [round(x, 3) for x in secs],
+                                            dm.get("f64", {}).get("sec_per_sweep", [])[:k])}
         line["dmrg"] = dm
+    if args.dmrg_big:
+        n_s, md_s = args.dmrg_big.split(":")
+        mds = DMRG_MAXDIM if md_s == "full" else [int(x) for x in md_s.split(",")]
+        forms = args.dmrg_big_forms.split(",")
+        db = {"workload": "QN Heisenberg N=%s open chain, MinDim = MaxDim = %s (the bond dimension follows the "
+                          "ramp to its maximum instead of saturating at the cutoff), cutoff 1e-10, niter 2, "
+                          "density-matrix bond step; H*psi in the named form" % (n_s, mds)}
+        for form in forms:
+            base, _, last = form.partition("+")
+            db[form] = dmrg_leg(int(n_s), mds, {"f64": 0, "f32": 1, "bf16x3": 2}[base], 2,
+                                1 if last == "f64last" else 0, mindim_follows=True)
+            print("dmrg_big leg %s done in %.1f s" % (form, db[form]["wall_s"]), file=sys.stderr, flush=True)
+        mixed_deltas(db, forms)
+        if args.phase_split:
+            db["phase_split_f64"] = dmrg_leg(int(n_s), mds, 0, 2, mindim_follows=True,
+                                             sync_timing=True)["last_sweep_split_s"]
+        line["dmrg_big"] = db
 
     if args.hubbard:
         geo, md_s = args.hubbard.split(":")
         nx, ny = (int(x) for x in geo.split("x"))
         mds = [int(x) for x in md_s.split(",")]
+        forms = args.hubbard_forms.split(",")
         hb = {"workload": "Hubbard %dx%d YPeriodic cylinder (squareLattice order), t=1, U=4, half filling, "
                           "(Nf, Sz) QNs, MPO from the host AutoMPO; maxdim %s, cutoff 1e-9, niter 2, noise 1e-6/1e-7 "
                           "on the first two sweeps; H*psi in the named form, everything else FP64" % (nx, ny, mds)}
-        for form in args.hubbard_forms.split(","):
+        for form in forms:
             base, _, last = form.partition("+")
             hb[form] = hubbard_leg(nx, ny, mds, {"f64": 0, "f32": 1, "bf16x3": 2}[base], 1 if last == "f64last" else 0)
             print("hubbard leg %s done in %.1f s" % (form, hb[form]["wall_s"]), file=sys.stderr, flush=True)
-        if "f64" in hb:
-            for form in args.hubbard_forms.split(","):
-                if form != "f64":
-                    hb["dE_%s_vs_f64" % form] = hb[form]["energy"] - hb["f64"]["energy"]
-                    hb["sweep_speedup_%s_vs_f64" % form] = round(hb["f64"]["wall_s"] / hb[form]["wall_s"], 3)
+        mixed_deltas(hb, forms)
+        if args.phase_split:
+            hb["phase_split_f64"] = hubbard_leg(nx, ny, mds, 0, sync_timing=True)["last_sweep_split_s"]
         line["hubbard"] = hb
 
     if not args.no_cpu:
         ref, t_cpu, threads, reps = cpu_baseline(psi, L, W, R)
         line["cpu_baseline"] = {"value": round(flops / t_cpu / 1e12, 5), "unit": "TFLOP/s",
-                                "cores": threads, "kind": "port",
-                                "sample": "oracle hpsi_dense (numpy/OpenBLAS) on the same chi=%d inputs, "
-                                          "median of %d after 1 warm-up, %.3f s per H*psi" % (chi, reps, t_cpu)}
+                                "cores": threads, "kind": "port", "cpu": cpu_model,
+                                "sample": "oracle hpsi_dense (numpy tensordot over the host BLAS) on the same "
+                                          "chi=%d inputs, median of %d after 1 warm-up, %.3f s per H*psi"
+                                          % (chi, reps, t_cpu)}
         nref = np.max(np.abs(ref))
         for prec, r in results.items():
             line["forms"][prec]["rel_err_vs_fp64_cpu"] = float(np.max(np.abs(r["out"] - ref)) / nref)

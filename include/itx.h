@@ -146,6 +146,12 @@ int itx_tune_size(itx_ctx *ctx);
 
 /* Algorithmic FP64-equivalent flops of one execution (sum of 2*m*n*k). */
 double itx_plan_flops(const itx_plan *plan);
+/* What a plan runs (diagnostic; no counterpart in the reference): the
+ * kernel's tile index (ITX_FORCE_TILE numbering; 5 = skinny, 10 = small
+ * blocks), the arithmetic form its kernel computes in (ITX_PREC_*: the
+ * HBM-bound skinny path computes in FP64 in every form), the number of
+ * block-pair tasks and of work units.  Any output pointer may be NULL. */
+int itx_plan_info(const itx_plan *plan, int *tile, int *form, int *ntasks, int *nunits);
 int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan);
 
 /* ---- H*psi chains replayed as a HIP graph -------------------------------

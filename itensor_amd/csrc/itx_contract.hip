@@ -629,6 +629,189 @@ __global__ __launch_bounds__(SM_T) void k_small(const double *__restrict__ A, co
   }
 }
 
+// Small-block contractions on the matrix cores: the FP32 and BF16x3 forms
+// of the small-block path (same 64 x 32 unit tiles, task/unit/table format,
+// split units and partial-tile layout as k_small).  Each k chunk of the A
+// and B tiles is staged in LDS zero-padded to the MFMA depth (and to the
+// full tile: rows and columns outside the block are zeros, so the padded
+// fragments add nothing); wave w computes rows 16w..16w+15 of the tile as
+// two 16 x 16 MFMA fragments.
+// FP32: operands rounded to FP32 at staging, v_mfma_f32_16x16x4f32.
+constexpr int SMF_LDA = SM_M + 17, SMF_LDB = SM_N + 17;  // odd pitches: k-strided staging hits distinct banks
+
+template <bool AKC, bool BKC>
+__global__ __launch_bounds__(SM_T) void k_small_f32(const double *__restrict__ A, const double *__restrict__ B,
+                                                    double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                    const WorkUnit *__restrict__ units, int nunits,
+                                                    const int64_t *__restrict__ tab, double alpha, double beta) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __shared__ float As[SM_K * SMF_LDA];
+  __shared__ float Bs[SM_K * SMF_LDB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const WorkUnit wu = units[u];
+    v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      const int rows = min(SM_M, tk.M - wu.m0), cols = min(SM_N, tk.N - wu.n0);
+      for (int k0 = 0; k0 < tk.K; k0 += SM_K) {
+        const int kc = min(SM_K, tk.K - k0), kcp = (kc + 3) & ~3;
+        __syncthreads();  // the previous chunk's tiles are no longer read
+        {
+          const int lead = AKC ? kcp : SM_M;
+          for (int e = tid; e < SM_M * kcp; e += SM_T) {
+            const int q = e / lead, r = e - q * lead;
+            const int m = AKC ? q : r, k = AKC ? r : q;
+            float v = 0.f;
+            if (m < rows && k < kc) v = float(A[tk.a_off + tab[tk.tAM + wu.m0 + m] + tab[tk.tAK + k0 + k]]);
+            As[k * SMF_LDA + m] = v;
+          }
+        }
+        {
+          const int lead = BKC ? kcp : SM_N;
+          for (int e = tid; e < SM_N * kcp; e += SM_T) {
+            const int q = e / lead, r = e - q * lead;
+            const int n = BKC ? q : r, k = BKC ? r : q;
+            float v = 0.f;
+            if (n < cols && k < kc) v = float(B[tk.b_off + tab[tk.tBK + k0 + k] + tab[tk.tBN + wu.n0 + n]]);
+            Bs[k * SMF_LDB + n] = v;
+          }
+        }
+        __syncthreads();
+        const int kl = lane >> 4, ml = 16 * wave + (lane & 15);
+        for (int kk = 0; kk < kcp; kk += 4) {
+          const float a = As[(kk + kl) * SMF_LDA + ml];
+          const float b0 = Bs[(kk + kl) * SMF_LDB + (lane & 15)];
+          const float b1 = Bs[(kk + kl) * SMF_LDB + 16 + (lane & 15)];
+          // D^T fragments: MFMA rows run over n, columns over m
+          acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a, acc[0], 0, 0, 0);
+          acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a, acc[1], 0, 0, 0);
+        }
+      }
+    }
+    const int ml = 16 * wave + (lane & 15);
+    if (wu.slot >= 0) {  // partial tile of a split unit (C is the workspace)
+      double *P = C + size_t(wu.slot) * (SM_M * SM_N);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) P[(16 * i + 4 * (lane >> 4) + r) * SM_M + ml] = double(acc[i][r]);
+      continue;
+    }
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int m = wu.m0 + ml;
+    if (m < t0.M) {
+      double *Cm = C + t0.c_off + tab[t0.tCM + m];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = wu.n0 + 16 * i + 4 * (lane >> 4) + r;
+          if (n < t0.N) {
+            double *cp = Cm + tab[t0.tCN + n];
+            double v = alpha * double(acc[i][r]);
+            if (beta != 0.0) v += beta * (*cp);
+            *cp = v;
+          }
+        }
+    }
+  }
+}
+
+// BF16x3: each operand value split at staging into x ~ hi + lo (the split
+// of k_gemm3), tiles [row][k] with k contiguous so that a lane's 8
+// consecutive k are one 16-byte LDS read; hi*lo + lo*hi + hi*hi on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  k chunks are padded to
+// the MFMA depth of 32.
+constexpr int SM3_LD = SM_K + 8;  // bf16 elements per LDS row (16-byte aligned rows)
+
+template <bool AKC, bool BKC>
+__global__ __launch_bounds__(SM_T) void k_small_bf16x3(const double *__restrict__ A, const double *__restrict__ B,
+                                                       double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                       const WorkUnit *__restrict__ units, int nunits,
+                                                       const int64_t *__restrict__ tab, double alpha,
+                                                       double beta) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __shared__ __align__(16) unsigned short Ah[SM_M * SM3_LD], Al[SM_M * SM3_LD];
+  __shared__ __align__(16) unsigned short Bh[SM_N * SM3_LD], Bl[SM_N * SM3_LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const WorkUnit wu = units[u];
+    v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      const int rows = min(SM_M, tk.M - wu.m0), cols = min(SM_N, tk.N - wu.n0);
+      for (int k0 = 0; k0 < tk.K; k0 += SM_K) {
+        const int kc = min(SM_K, tk.K - k0), kcp = (kc + 31) & ~31;
+        __syncthreads();
+        {
+          const int lead = AKC ? kcp : SM_M;
+          for (int e = tid; e < SM_M * kcp; e += SM_T) {
+            const int q = e / lead, r = e - q * lead;
+            const int m = AKC ? q : r, k = AKC ? r : q;
+            unsigned short h = 0, l = 0;
+            if (m < rows && k < kc) split_bf16(A[tk.a_off + tab[tk.tAM + wu.m0 + m] + tab[tk.tAK + k0 + k]], h, l);
+            Ah[m * SM3_LD + k] = h;
+            Al[m * SM3_LD + k] = l;
+          }
+        }
+        {
+          const int lead = BKC ? kcp : SM_N;
+          for (int e = tid; e < SM_N * kcp; e += SM_T) {
+            const int q = e / lead, r = e - q * lead;
+            const int n = BKC ? q : r, k = BKC ? r : q;
+            unsigned short h = 0, l = 0;
+            if (n < cols && k < kc) split_bf16(B[tk.b_off + tab[tk.tBK + k0 + k] + tab[tk.tBN + wu.n0 + n]], h, l);
+            Bh[n * SM3_LD + k] = h;
+            Bl[n * SM3_LD + k] = l;
+          }
+        }
+        __syncthreads();
+        const int k8 = 8 * (lane >> 4), ml = 16 * wave + (lane & 15);
+        for (int kk = 0; kk < kcp; kk += 32) {
+          const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&Ah[ml * SM3_LD + kk + k8]);
+          const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&Al[ml * SM3_LD + kk + k8]);
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const int nl = 16 * i + (lane & 15);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bh[nl * SM3_LD + kk + k8]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Bl[nl * SM3_LD + kk + k8]);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[i], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[i], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[i], 0, 0, 0);
+          }
+        }
+      }
+    }
+    const int ml = 16 * wave + (lane & 15);
+    if (wu.slot >= 0) {
+      double *P = C + size_t(wu.slot) * (SM_M * SM_N);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) P[(16 * i + 4 * (lane >> 4) + r) * SM_M + ml] = double(acc[i][r]);
+      continue;
+    }
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int m = wu.m0 + ml;
+    if (m < t0.M) {
+      double *Cm = C + t0.c_off + tab[t0.tCM + m];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = wu.n0 + 16 * i + 4 * (lane >> 4) + r;
+          if (n < t0.N) {
+            double *cp = Cm + tab[t0.tCN + n];
+            double v = alpha * double(acc[i][r]);
+            if (beta != 0.0) v += beta * (*cp);
+            *cp = v;
+          }
+        }
+    }
+  }
+}
+
 static_assert(SM_T_RED == SM_T, "reduction pass block size");
 
 // Reduction pass of split plans: unit u = {task (for C's tables), first
@@ -851,6 +1034,7 @@ constexpr TileCfg kTiles[] = {
 };
 constexpr int kNumTiles = 14;
 constexpr int kSkinny = 5;
+static_assert(kSkinny == kSkinnyTile, "tile index of the skinny path");
 constexpr int kSmall = 10;
 constexpr int kSkinnyOcc = 4;
 
@@ -915,8 +1099,20 @@ const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block) {
     *block = SK_ROWS;
     return reinterpret_cast<const void *>(&k_skinny);
   }
-  if (tile == kSmall) {  // latency-bound: FP64 arithmetic in every form
+  if (tile == kSmall) {  // FP64 on the VALU, FP32 / BF16x3 on the matrix cores
     *block = SM_T;
+    if (prec == ITX_PREC_F32) {
+      if (akc) return bkc ? reinterpret_cast<const void *>(&k_small_f32<true, true>)
+                          : reinterpret_cast<const void *>(&k_small_f32<true, false>);
+      return bkc ? reinterpret_cast<const void *>(&k_small_f32<false, true>)
+                 : reinterpret_cast<const void *>(&k_small_f32<false, false>);
+    }
+    if (prec == ITX_PREC_BF16X3) {
+      if (akc) return bkc ? reinterpret_cast<const void *>(&k_small_bf16x3<true, true>)
+                          : reinterpret_cast<const void *>(&k_small_bf16x3<true, false>);
+      return bkc ? reinterpret_cast<const void *>(&k_small_bf16x3<false, true>)
+                 : reinterpret_cast<const void *>(&k_small_bf16x3<false, false>);
+    }
     if (akc) return bkc ? reinterpret_cast<const void *>(&k_small<true, true>)
                         : reinterpret_cast<const void *>(&k_small<true, false>);
     return bkc ? reinterpret_cast<const void *>(&k_small<false, true>)
@@ -1537,6 +1733,17 @@ int itx_plan_execute(itx_ctx *ctx, const itx_plan *plan, const double *A, const 
 }
 
 double itx_plan_flops(const itx_plan *plan) { return plan ? plan->flops : 0.0; }
+
+int itx_plan_info(const itx_plan *plan, int *tile, int *form, int *ntasks, int *nunits) {
+  if (!plan) return ITX_ERR_ARG;
+  // the skinny VALU path is HBM-bound and computes in FP64 in every form
+  const int f = plan->tile == itx::kSkinnyTile ? ITX_PREC_F64 : plan->prec;
+  if (tile) *tile = plan->tile;
+  if (form) *form = f;
+  if (ntasks) *ntasks = plan->ntasks;
+  if (nunits) *nunits = plan->nunits;
+  return ITX_OK;
+}
 
 int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan) {
   if (!plan) return ITX_OK;

@@ -147,6 +147,7 @@ KernelArgs plan_args(const itx_plan *p, int which, const double *A, const double
 constexpr int SM_T_RED = 256;  // workgroup size of the reduction pass
 
 constexpr int RED_SLOTS = 2048;  // partial-sum slots in ctx->red
+constexpr int kSkinnyTile = 5;   // tile index of the skinny (FP64 VALU) path
 
 }  // namespace itx
 

@@ -503,7 +503,7 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
     check(itx_plan_dense(ctx(), int(dA.size()), dA.data(), la.data(), int(dB.size()), dB.data(), lb.data(),
                          int(dC.size()), dC.data(), lc.data(), prec, &plan_),
           "itx_plan_dense");
-    flops_ = itx_plan_flops(plan_);
+    account(plan_);
     return;
   }
   QNTaskList tl = qnTaskList(A.inds(), A.store().offsets, B.inds(), B.store().offsets);
@@ -554,7 +554,7 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
     check(itx_plan_qn(ctx(), rA, tl.la.data(), rB, tl.lb.data(), rC, tl.lc.data(), tl.nt, tl.offs.data(),
                       tl.bda.data(), tl.bdb.data(), prec, &plan_),
           "itx_plan_qn");
-    flops_ = itx_plan_flops(plan_);
+    account(plan_);
     return;
   }
   try {
@@ -565,12 +565,20 @@ ContractPlan::ContractPlan(ITensor const &A, ITensor const &B, int prec, bool pe
             "itx_plan_qn");
       blockPlans_.push_back(p);
       blockBeta_.push_back(tl.cfirst[size_t(t)] ? 0.0 : 1.0);
-      flops_ += itx_plan_flops(p);
+      account(p);
     }
   } catch (...) {
     for (auto *p : blockPlans_) itx_plan_destroy(ctx(), p);
     throw;
   }
+}
+
+void ContractPlan::account(itx_plan const *p) {
+  const double f = itx_plan_flops(p);
+  int form = ITX_PREC_F64;
+  check(itx_plan_info(p, nullptr, &form, nullptr, nullptr), "itx_plan_info");
+  flops_ += f;
+  formFlops_[size_t(form)] += f;
 }
 
 ContractPlan::~ContractPlan() {

@@ -10,6 +10,7 @@
 // does on the CPU (labels, QN block lists, truncation decisions).
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -383,6 +384,8 @@ class ContractPlan {
   BlockOffsets const &offsets() const { return coffs_; }
   size_t size() const { return size_; }
   double flops() const { return flops_; }
+  // flops per arithmetic form its kernels compute in (ITX_PREC_* index)
+  std::array<double, 3> const &flopsByForm() const { return formFlops_; }
   itx_plan *plan() const { return plan_; }  // null in per-block mode
   bool perBlock() const { return !blockPlans_.empty() || (perBlock_ && !plan_); }
   std::shared_ptr<Storage> newOutput() const;
@@ -396,6 +399,8 @@ class ContractPlan {
   BlockOffsets coffs_;
   size_t size_ = 0;
   double flops_ = 0;
+  std::array<double, 3> formFlops_{};
+  void account(itx_plan const *p);
   itx_plan *plan_ = nullptr;
   std::vector<itx_plan *> blockPlans_;
   std::vector<double> blockBeta_;

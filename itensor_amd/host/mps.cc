@@ -132,6 +132,7 @@ struct LocalOp::Program {
   IndexSet outInds;
   itx_chain *chain = nullptr;
   double flops = 0;
+  std::array<double, 3> formFlops{};
   ~Program() {
     // after shutdown() the context has released the chain already
     if (chain && liveCtx()) itx_chain_destroy(liveCtx(), chain);
@@ -185,6 +186,7 @@ LocalOp::Program *LocalOp::program(ITensor const &phi) const {
                                                          cover ? &cperm : nullptr));
     ContractPlan const &cp = *prg->stages.back();
     prg->flops += cp.flops();
+    for (int f = 0; f < 3; ++f) prg->formFlops[size_t(f)] += cp.flopsByForm()[size_t(f)];
     auto st = std::make_shared<Storage>();
     st->type = cp.isQN() ? SType::QDense : SType::Dense;
     st->offsets = cp.offsets();
@@ -244,6 +246,13 @@ double LocalOp::flops(ITensor const &phi) const {
   Program *prg = program(phi);
   if (!prg) throw ITError("LocalOp::flops: chain cannot be planned for this phi (Eager mode or mixed storage)");
   return prg->flops;
+}
+
+std::array<double, 3> LocalOp::flopsByForm(ITensor const &phi) const {
+  PrecisionScope ps(prec_);
+  Program *prg = program(phi);
+  if (!prg) return {};
+  return prg->formFlops;
 }
 
 std::pair<ITensor, double> LocalOp::deltaRho(ITensor const &AA, ITensor const &combine, Direction dir) const {
@@ -446,12 +455,17 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
       PH.fuseOps(phi.inds());
       DavidsonStats st;
       energy = davidson(PH, phi, args, &st);
+      std::array<double, 3> hf{};
+      if (obs) {
+        hf = PH.flopsByForm(phi);
+        for (auto &x : hf) x *= st.products;
+      }
       auto t2 = mark();
       auto spec = svdBond(psi, b, phi, ha == 1 ? Fromleft : Fromright, args, &PH);
       auto t3 = mark();
       if (obs) {
         BondRecord r{int(sw + 1), b, ha, energy, spec.truncerr, commonIndex(psi(b), psi(b + 1)).dim(), st.products,
-                     secs(t0, t3), secs(t0, t1), secs(t1, t2), secs(t2, t3)};
+                     secs(t0, t3), secs(t0, t1), secs(t1, t2), secs(t2, t3), hf};
         obs(r);
       }
     }

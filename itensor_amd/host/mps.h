@@ -92,6 +92,9 @@ class LocalOp {
   // algorithmic flops (sum of 2mnk over every block pair) of one product
   // on a tensor with phi's structure (plans the chain if needed)
   double flops(ITensor const &phi) const;
+  // the same flops split by the arithmetic form the kernels compute in
+  // (ITX_PREC_* index)
+  std::array<double, 3> flopsByForm(ITensor const &phi) const;
   // noise correction to the density matrix of the bond (localop.h:375-406):
   // X = combine * noPrime(AA*L*Op1) (Fromleft) or noPrime(AA*R*Op2)
   // (Fromright), drho = (X X^dag + its transpose) / 2; returns drho and its
@@ -123,6 +126,7 @@ class LocalMPO {
   void setMode(HpsiMode m) { lop_.setMode(m); }
   void setCoverOutput(bool on) { lop_.setCoverOutput(on); }
   double flops(ITensor const &phi) const { return lop_.flops(phi); }
+  std::array<double, 3> flopsByForm(ITensor const &phi) const { return lop_.flopsByForm(phi); }
   std::pair<ITensor, double> deltaRho(ITensor const &AA, ITensor const &combine, Direction dir) const {
     return lop_.deltaRho(AA, combine, dir);
   }
@@ -167,6 +171,9 @@ struct BondRecord {
   // wall time of the environment update (position), davidson and svdBond;
   // device work is attributed to its phase only with Args "SyncTiming"
   double t_env = 0, t_davidson = 0, t_svd = 0;
+  // algorithmic flops of the bond's H*psi products by the arithmetic form
+  // their kernels computed in (ITX_PREC_* index; zero in Eager mode)
+  std::array<double, 3> hpsi_flops{};
 };
 
 // MPS::svdBond (mps/mps_impl.h:21-99).  With Args "UseSVD", or with no

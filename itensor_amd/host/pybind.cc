@@ -76,6 +76,7 @@ py::dict recordDict(BondRecord const &r) {
   d["t_env"] = r.t_env;
   d["t_davidson"] = r.t_davidson;
   d["t_svd"] = r.t_svd;
+  d["hpsi_flops"] = std::vector<double>(r.hpsi_flops.begin(), r.hpsi_flops.end());
   return d;
 }
 
@@ -272,6 +273,11 @@ PYBIND11_MODULE(_itensor_host, m) {
         o.prepare(phi);
         return o.op.flops(phi);
       })
+      .def("flops_by_form", [](PyLocalOp &o, ITensor const &phi) {
+        o.prepare(phi);
+        auto f = o.op.flopsByForm(phi);
+        return std::vector<double>(f.begin(), f.end());
+      }, "flops of one product by the form its kernels compute in: [f64, f32, bf16x3]")
       .def("delta_rho",
            [](PyLocalOp &o, ITensor const &AA, ITensor const &combine, bool fromleft) {
              auto r = o.op.deltaRho(AA, combine, fromleft ? Fromleft : Fromright);

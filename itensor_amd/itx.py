@@ -57,6 +57,8 @@ _PROTOS = {
                               _i64p, _i64p, C.c_int, C.POINTER(_vp)]),
     "itx_plan_execute": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double]),
     "itx_plan_flops": (C.c_double, [_vp]),
+    "itx_plan_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
     "itx_plan_destroy": (C.c_int, [_vp, _vp]),
     "itx_dot": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _dp]),
     "itx_nrm2": (C.c_int, [_vp, _vp, C.c_size_t, _dp]),
@@ -308,6 +310,15 @@ class Plan:
     @property
     def flops(self) -> float:
         return self.ctx.L.itx_plan_flops(self.h)
+
+    @property
+    def info(self) -> dict:
+        """tile index, arithmetic form, block-pair tasks and work units."""
+        t, f, nt, nu = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.L.itx_plan_info(self.h, C.byref(t), C.byref(f), C.byref(nt), C.byref(nu)),
+                       "itx_plan_info")
+        return {"tile": t.value, "form": {v: k for k, v in PREC.items()}[f.value], "ntasks": nt.value,
+                "nunits": nu.value}
 
     def execute(self, A: int, B: int, Cp: int, alpha=1.0, beta=0.0):
         self.ctx.check(self.ctx.L.itx_plan_execute(self.ctx.h, self.h, _vp(A), _vp(B), _vp(Cp),

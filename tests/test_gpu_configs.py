@@ -1,0 +1,232 @@
+"""GPU parity at the full sizes of the benchmark configurations.
+
+* Dense two-site H*psi at chi = 256, 512, 1024 (d = 2, k = 5, Heisenberg
+  MPO) in the FP64, FP32 and BF16x3 forms, through the fused chain the
+  benchmark times (psi*L, *W12, *R) and through the four-stage chain,
+  against the oracle's hpsi_dense (FP64 numpy); at chi = 256 every kernel
+  tile forced in turn (ITX_FORCE_TILE), and the tile the benchmark's
+  autotuner picks at chi = 1024.
+* Block-sparse H*psi at chi = 1024 over 31 Sz sectors in every execution
+  mode and form against the oracle's local_op_product: block lists
+  bit-exact, values to the form's tolerance, and the flops each form's
+  kernels compute in that form.
+* QN Heisenberg DMRG, N = 100, maxdim ramp 10 -> 2048 over 10 sweeps,
+  cutoff 1e-10: the FP64 energy against the known ground energy of the
+  chain, the mixed forms against the FP64 run.
+* Hubbard 8 x 4 cylinder DMRG at the benchmark's schedule: mixed forms
+  against the FP64 run.
+
+Tolerances (relative to max |H psi|): FP64 1e-12, FP32 2e-5, BF16x3 1e-4.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from itensor_amd.workloads import (DMRG_MAXDIM, FUSED, HUBBARD_MAXDIM, STAGES, fused_dims, heisenberg_qn_chain,
+                                   hubbard_cylinder, stage_dims)
+from oracle.dense import heisenberg_w, hpsi_dense
+from oracle.dmrg import local_op_product
+from tests.bridge import cten, same_inds
+from tests.hpsi_cases import SECTORS_1024, hpsi_operands
+
+pytestmark = pytest.mark.gpu
+TOL = {"f64": 1e-12, "f32": 2e-5, "bf16x3": 1e-4}
+FORMS = ("f64", "f32", "bf16x3")
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+# ground energy of the open S=1/2 Heisenberg chain at N = 100 (J = Jz = 1)
+E_HEIS_100 = -44.127739860
+
+
+@pytest.fixture(scope="module")
+def actx(torch_dev):
+    """A context with the tile autotuner on, as the benchmark runs."""
+    import torch
+
+    from itensor_amd.itx import Context
+
+    c = Context(0, torch.cuda.current_stream().cuda_stream)
+    c.set_autotune(True)
+    yield c
+    torch.cuda.synchronize()
+    c.close()
+
+
+_DENSE = {}
+
+
+def _dense_case(chi):
+    if chi not in _DENSE:
+        rng = np.random.default_rng(20261202 + 2 + chi)
+        psi = rng.standard_normal((chi, 2, 2, chi))
+        L = rng.standard_normal((chi, 5, chi))
+        R = rng.standard_normal((chi, 5, chi))
+        W = heisenberg_w()
+        _DENSE[chi] = (psi, L, W, R, hpsi_dense(psi, L, W, W, R))
+    return _DENSE[chi]
+
+
+def _chain(ctx, dev, chi, prec, chain, tile=None):
+    """Run one H*psi through libitx plans; returns (result, plan infos)."""
+    psi, L, W, R, _ = _dense_case(chi)
+    if chain == "fused":
+        W12 = np.einsum("aijb,bklc->aijklc", W, W)  # (h0,s1,s1',s2,s2',h2)
+        dims, labels, ops = fused_dims(chi), FUSED, [L, W12, R]
+    else:
+        dims, labels, ops = stage_dims(chi), STAGES, [L, W, W, R]
+    old = os.environ.get("ITX_FORCE_TILE")
+    if tile is not None:
+        os.environ["ITX_FORCE_TILE"] = str(tile)
+    try:
+        plans = [ctx.plan_dense(da, la, db, lb, dc, lc, prec) for (da, db, dc), (la, lb, lc) in zip(dims, labels)]
+    finally:
+        if tile is not None:
+            if old is None:
+                del os.environ["ITX_FORCE_TILE"]
+            else:
+                os.environ["ITX_FORCE_TILE"] = old
+    x = dev.put(psi)
+    for p, op, (_, _, dc) in zip(plans, ops, dims):
+        y = dev.empty(int(np.prod(dc)))
+        p.execute(x.data_ptr(), dev.put(op).data_ptr(), y.data_ptr())
+        x = y
+    out = dev.get(x, (chi, 2, 2, chi))
+    infos = [p.info for p in plans]
+    for p in plans:
+        p.close()
+    return out, infos
+
+
+def _rel(a, ref):
+    return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("chi", [256, 512, 1024])
+@pytest.mark.parametrize("prec", FORMS)
+def test_config2_dense_hpsi_fused_chain(actx, dev, chi, prec):
+    out, infos = _chain(actx, dev, chi, prec, "fused")
+    err = _rel(out, _dense_case(chi)[4])
+    print(f"[config2] chi={chi} form={prec} tiles={[i['tile'] for i in infos]} "
+          f"forms={[i['form'] for i in infos]} rel err {err:.2e}", flush=True)
+    assert err < TOL[prec]
+    # S1 and S4 (98% of the flops) compute in the requested form
+    assert infos[0]["form"] == prec and infos[2]["form"] == prec
+
+
+@pytest.mark.parametrize("prec", FORMS)
+def test_config2_dense_hpsi_four_stage_chain(actx, dev, prec):
+    out, _ = _chain(actx, dev, 256, prec, "4stage")
+    assert _rel(out, _dense_case(256)[4]) < TOL[prec]
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("prec", FORMS)
+def test_config2_every_tile_forced(ctx, dev, tile, prec):
+    out, infos = _chain(ctx, dev, 256, prec, "fused", tile=tile)
+    err = _rel(out, _dense_case(256)[4])
+    print(f"[config2 tile] tile={tile} form={prec} rel err {err:.2e}", flush=True)
+    assert [i["tile"] for i in infos] == [tile] * 3
+    assert err < TOL[prec]
+
+
+def test_config2_headline_tile_at_chi1024(ctx, dev):
+    """The 8-wave 128 x 128 tile with 32-deep k stages (tile 11), which the
+    autotuner picks for S1 / S4 at chi = 1024 in FP64, forced."""
+    out, infos = _chain(ctx, dev, 1024, "f64", "fused", tile=11)
+    assert infos[0]["tile"] == 11
+    assert _rel(out, _dense_case(1024)[4]) < TOL["f64"]
+
+
+@pytest.fixture(scope="module")
+def H(torch_dev):
+    from itensor_amd import host
+
+    return host
+
+
+@pytest.fixture(scope="module")
+def qn_case():
+    phi, L, W1, W2, R = hpsi_operands(SECTORS_1024, seed=20261202 + 3)
+    return phi, L, W1, W2, R, local_op_product(phi, L, W1, W2, R)
+
+
+@pytest.mark.parametrize("prec", FORMS)
+@pytest.mark.parametrize("mode", ["graph", "planned", "eager", "per_block"])
+def test_config3_qn_hpsi_chi1024(H, qn_case, mode, prec):
+    phi, L, W1, W2, R, ref = qn_case
+    m = {"graph": H.HPSI_GRAPH, "planned": H.HPSI_PLANNED, "eager": H.HPSI_EAGER,
+         "per_block": H.HPSI_PER_BLOCK}[mode]
+    op = H.LocalOp(cten(W1), cten(W2), cten(L), cten(R), True)
+    op.set_mode(m)
+    op.set_precision(PREC[prec])
+    got = op.product(cten(phi))
+    assert same_inds(got, cten(ref))
+    assert [(tuple(b), o) for b, o in got.offsets] == [(b, o) for b, o in ref.offsets]
+    err = float(np.max(np.abs(got.store() - ref.store)) / np.max(np.abs(ref.store)))
+    msg = f"[config3] mode={mode} form={prec} blocks={len(ref.offsets)} rel err {err:.2e}"
+    if m != H.HPSI_EAGER:
+        f = op.flops_by_form(cten(phi))
+        msg += f" flops by form f64/f32/bf16x3 = {[round(x / sum(f), 4) for x in f]}"
+        # everything but the HBM-bound MPO stage computes in the named form
+        assert f[PREC[prec]] >= 0.9 * sum(f)
+    print(msg, flush=True)
+    assert err < TOL[prec]
+
+
+_LEGS = {}
+
+
+def _dmrg_leg(H, form):
+    if form not in _LEGS:
+        base, _, last = form.partition("+")
+        mpo, A = heisenberg_qn_chain(H, 100)
+        sw = [{"MaxDim": m, "Cutoff": 1e-10, "Niter": 2} for m in DMRG_MAXDIM]
+        E, recs, _, secs = H.dmrg(mpo, A, sw, {"HpsiPrecision": PREC[base], "FinalFP64Sweeps": 1 if last else 0})
+        fl = np.sum([r["hpsi_flops"] for r in recs], axis=0)
+        _LEGS[form] = (E, recs, secs, fl)
+        print(f"[config4] form={form} E={E:.12f} max dim {max(r['dim'] for r in recs)} "
+              f"sec/sweep {[round(x, 3) for x in secs]} hpsi flops by form {fl / fl.sum()}", flush=True)
+    return _LEGS[form]
+
+
+def test_config4_fp64_energy(H):
+    E, recs, secs, fl = _dmrg_leg(H, "f64")
+    assert len(recs) == len(DMRG_MAXDIM) * 2 * 99
+    assert abs(E - E_HEIS_100) < 1e-9
+    assert fl[0] == fl.sum()
+
+
+@pytest.mark.parametrize("form", ["f32", "bf16x3", "f32+f64last", "bf16x3+f64last"])
+def test_config4_mixed_energy_gate(H, form):
+    E64 = _dmrg_leg(H, "f64")[0]
+    E, recs, secs, fl = _dmrg_leg(H, form)
+    base = form.partition("+")[0]
+    # the mixed sweeps run their H*psi in the named form
+    assert fl[PREC[base]] >= 0.5 * fl.sum()
+    print(f"[config4] {form}: dE = {E - E64:.3e}", flush=True)
+    assert abs(E - E64) <= 1e-9
+
+
+_HLEGS = {}
+
+
+def _hubbard_leg(H, form):
+    if form not in _HLEGS:
+        base, _, last = form.partition("+")
+        mpo, A = hubbard_cylinder(H, 8, 4, 1.0, 4.0, True)
+        noise = [1e-6, 1e-7] + [0.0] * (len(HUBBARD_MAXDIM) - 2)
+        sw = [{"MaxDim": m, "Cutoff": 1e-9, "Niter": 2, "Noise": z} for m, z in zip(HUBBARD_MAXDIM, noise)]
+        E, recs, _, secs = H.dmrg(mpo, A, sw, {"HpsiPrecision": PREC[base], "FinalFP64Sweeps": 1 if last else 0})
+        fl = np.sum([r["hpsi_flops"] for r in recs], axis=0)
+        _HLEGS[form] = (E, recs, secs, fl)
+        print(f"[config5] form={form} E={E:.12f} max dim {max(r['dim'] for r in recs)} "
+              f"sec/sweep {[round(x, 3) for x in secs]} hpsi flops by form {fl / fl.sum()}", flush=True)
+    return _HLEGS[form]
+
+
+@pytest.mark.parametrize("form", ["f32", "bf16x3", "f32+f64last", "bf16x3+f64last"])
+def test_config5_hubbard_mixed_energy_gate(H, form):
+    E64 = _hubbard_leg(H, "f64")[0]
+    E, recs, secs, fl = _hubbard_leg(H, form)
+    print(f"[config5] {form}: dE = {E - E64:.3e}", flush=True)
+    assert abs(E - E64) <= 1e-9

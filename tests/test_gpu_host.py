@@ -349,7 +349,7 @@ def test_hubbard_dmrg_matches_ed(H, Nx, Ny, U):
     Sz = 0 sector; 6 sites are exact at bond dimension 64.  The first
     sweeps carry the density-matrix noise term (as sample/dmrg.cc does),
     which moves weight into the sectors a product start state lacks."""
-    import bench
+    from itensor_amd import workloads as bench
     from oracle import hubbard as OH
 
     N = Nx * Ny
@@ -466,7 +466,7 @@ def test_full_block_krylov_same_trajectory(H):
     to 1e-10 and the same bond dimensions and truncation errors; the added
     blocks are exact zeros, so only the summation order of the grouped
     contractions can differ."""
-    import bench
+    from itensor_amd import workloads as bench
 
     runs = []
     for full in (True, False):
@@ -481,3 +481,101 @@ def test_full_block_krylov_same_trajectory(H):
         assert a["dim"] == b["dim"], (a, b)
         assert abs(a["energy"] - b["energy"]) < 1e-10 * max(1.0, abs(b["energy"])), (a, b)
         assert abs(a["truncerr"] - b["truncerr"]) < 1e-12 + 1e-6 * b["truncerr"], (a, b)
+
+
+@pytest.mark.parametrize("full", [True, False])
+def test_qn_dmrg_mps_block_lists_match_oracle(H, full):
+    """QN DMRG N=12 with the Krylov vectors on phi's full block set (the
+    default) and without: after five sweeps every MPS tensor has the
+    oracle's block list (blocks and offsets), and every bond the oracle's
+    kept dimension."""
+    N = 12
+    sites = OD.spin_half_sites(N, True)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)])
+    sw = [{"MaxDim": m, "Cutoff": 1e-12, "Niter": 2} for m in (10, 20, 40, 60, 60)]
+    trace = []
+    Eo, psio = OD.dmrg(Hm, psi0, sw, record=trace.append)
+    Eg, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {"FullBlockKrylov": full})
+    assert [r["dim"] for r in recs] == [t["dim"] for t in trace]
+    got = [[(tuple(b), o) for b, o in t.offsets] for t in tens]
+    want = [[(tuple(b), o) for b, o in a.offsets] for a in psio.A]
+    assert got == want
+    assert abs(Eg - Eo) < 1e-10
+
+
+def test_qn_dmrg_svd_route_matches_oracle(H):
+    """Cutoff 1e-13 with no noise: every bond step of the reference takes
+    the svd route (mps_impl.h:50-65); the host layer follows the oracle's
+    per-bond energies to 1e-10 and its kept dimensions exactly."""
+    N = 10
+    sites = OD.spin_half_sites(N, True)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)])
+    sw = [{"MaxDim": m, "Cutoff": 1e-13, "Niter": 2} for m in (10, 20, 40, 40, 40)]
+    trace = []
+    Eo, _ = OD.dmrg(Hm, psi0, sw, record=trace.append)
+    assert {t["route"] for t in trace} == {"svd"}
+    Eg, recs, tens, secs = H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, {})
+    assert [r["dim"] for r in recs] == [t["dim"] for t in trace]
+    assert max(abs(r["energy"] - t["energy"]) for r, t in zip(recs, trace)) < 1e-10
+    assert abs(Eg - GOLD["energies"]["10"]) < 1e-10
+
+
+@pytest.mark.parametrize("qn", [False, True])
+@pytest.mark.parametrize("fromleft", [True, False])
+def test_svd_bond_routes_match_oracle(H, qn, fromleft):
+    """svdBond on a two-site wavefunction in both routes: the kept spectrum
+    and truncation error as the oracle's, the product of the new site
+    tensors equal to the oracle's, the centre normalised."""
+    N = 6
+    sites = OD.spin_half_sites(N, qn)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)]) if qn else OD.random_mps(sites, 4, seed=3)
+    _, psi = OD.dmrg(Hm, psi0, [{"MaxDim": 8, "Cutoff": 1e-10}] * 2)
+    b = 3
+    phi = psi(b) * psi(b + 1)
+    for cutoff, route in ((1e-13, "svd"), (1e-10, "denmat")):
+        args = {"Cutoff": cutoff, "MaxDim": 6, "RespectDegenerate": True, "DoNormalize": True}
+        Ao, Bo, so = OD.svd_bond(psi(b), psi(b + 1), phi, fromleft, args)
+        assert so["route"] == route
+        Ag, Bg, eigs, terr = H.svd_bond([cten(a) for a in psi.A], b, cten(phi), fromleft, args)
+        assert np.allclose(np.sort(eigs)[::-1], np.sort(so["eigs"])[::-1], rtol=0, atol=1e-12)
+        assert abs(terr - so["truncerr"]) < 1e-14
+        prod_o = Ao * Bo
+        prod_g = oten(Ag * Bg)
+        perm = [[(i.id, i.plev) for i in prod_g.inds].index((j.id, j.plev)) for j in prod_o.inds]
+        assert _rel(np.transpose(prod_g.to_dense_array(), perm), prod_o.to_dense_array()) < 1e-12
+        centre = Bg if fromleft else Ag
+        assert abs(H.norm(centre) - 1.0) < 1e-13
+
+
+def test_position_orthogonalises_without_changing_the_state(H):
+    """MPS::position(1) (orthMPS route) on an MPS that is not orthogonal
+    (rightLim = N+1): the state is unchanged (to 1e-12), sites 2..N become
+    right-orthogonal, and the limits become (0, 2)."""
+    N = 6
+    sites = OD.spin_half_sites(N, False)
+    psi = OD.random_mps(sites, 4, seed=9)
+    rng = np.random.default_rng(4)
+    A = [a.copy() for a in psi.A]
+    for a in A:  # break the orthogonality, keep the structure
+        a.d = a.d * (1.0 + 0.3 * rng.standard_normal(a.d.shape))
+
+    def full_state(ts):
+        x = ts[0]
+        for t in ts[1:]:
+            x = x * t
+        return x
+
+    ref = full_state(A)
+    tens, ll, rl = H.position([cten(a) for a in A], 1, 0, N + 1)
+    assert (ll, rl) == (0, 2)
+    got = oten(full_state(tens))
+    perm = [[(i.id, i.plev) for i in got.inds].index((j.id, j.plev)) for j in ref.inds]
+    assert _rel(np.transpose(got.to_dense_array(), perm), ref.to_dense_array()) < 1e-12
+    for j in range(1, N):
+        T = oten(tens[j])
+        lk = [I for I in T.inds if "Link" in I.tags and any(I.id == K.id for K in oten(tens[j - 1]).inds)][0]
+        G = T * T.prime(1, lk).dag()
+        assert np.allclose(G.to_dense_array(), np.eye(lk.dim), atol=1e-12)
