@@ -13,6 +13,7 @@
 #include <array>
 #include <cstdint>
 #include <functional>
+#include <iosfwd>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -405,6 +406,19 @@ class ContractPlan {
   std::vector<itx_plan *> blockPlans_;
   std::vector<double> blockBeta_;
 };
+
+// ------------------------------------------------------------ file I/O --
+// Binary stream format of the reference's write/read (index.cc:711-745,
+// itensor.cc:768-806, itdata/qdense.h:236-248): real Dense and QDense
+// storage, U(1) quantum numbers, tags of at most 7 characters.
+void write(std::ostream &s, Index const &I);
+Index readIndex(std::istream &s);
+void write(std::ostream &s, ITensor const &T);
+ITensor readITensor(std::istream &s);
+void read(std::istream &s, ITensor &T);
+// writeToFile / readFromFile (util/readwrite.h:336-370)
+void writeToFile(std::string const &fname, ITensor const &T);
+ITensor readITensorFromFile(std::string const &fname);
 
 // ------------------------------------------------------------- decomps --
 struct Spectrum {

@@ -36,6 +36,13 @@ class MPS {
 
 using MPO = std::vector<ITensor>;  // W_1..W_N (0-based storage)
 
+// MPS::write / read (mps/mps.cc:353-378): length, the N+2 tensors of the
+// reference's 1-based store, the orthogonality limits
+void write(std::ostream &s, MPS const &psi);
+MPS readMPS(std::istream &s);
+void writeToFile(std::string const &fname, MPS const &psi);
+MPS readMPSFromFile(std::string const &fname);
+
 // How LocalOp::product runs its contraction chain.
 //   Eager    : ITensor operator* per stage (planned on every call, plans
 //              cached by structure inside libitx)

@@ -4,6 +4,8 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <sstream>
+
 #include "autompo.h"
 #include "mps.h"
 
@@ -247,6 +249,26 @@ PYBIND11_MODULE(_itensor_host, m) {
   }, py::arg("H"), py::arg("A"), py::arg("sweeps"), py::arg("args"), py::arg("left_lim") = 0,
      py::arg("right_lim") = 2,
      "dmrg(H, psi, sweeps, args) (mps/dmrg.h:51-76); left_lim / right_lim are psi's orthogonality limits");
+  m.def("write_itensor", [](std::string const &path, ITensor const &T) { writeToFile(path, T); },
+        "writeToFile(path, T) in the reference's binary format (itensor.cc:768-806)");
+  m.def("read_itensor", &readITensorFromFile, "readFromFile<ITensor>(path)");
+  m.def("itensor_bytes", [](ITensor const &T) {
+    std::ostringstream os(std::ios::binary);
+    write(os, T);
+    return py::bytes(os.str());
+  }, "the bytes ITensor::write puts on a stream");
+  m.def("itensor_from_bytes", [](py::bytes b) {
+    std::istringstream is(std::string(b), std::ios::binary);
+    return readITensor(is);
+  });
+  m.def("write_mps", [](std::string const &path, std::vector<ITensor> const &A, int ll, int rl) {
+    writeToFile(path, MPS(A, ll, rl));
+  }, py::arg("path"), py::arg("A"), py::arg("left_lim") = 0, py::arg("right_lim") = 2,
+     "MPS::write to a file (mps/mps.cc:365-378)");
+  m.def("read_mps", [](std::string const &path) {
+    MPS psi = readMPSFromFile(path);
+    return py::make_tuple(psi.tensors(), psi.leftLim(), psi.rightLim());
+  }, "MPS::read from a file: (tensors, leftLim, rightLim)");
   m.def("position", [](std::vector<ITensor> const &A, int i, int left_lim, int right_lim, py::dict args) {
     MPS psi(A, left_lim, right_lim);
     position(psi, i, argsFrom(args));
