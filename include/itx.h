@@ -80,6 +80,18 @@ int itx_trim(itx_ctx *ctx);
 int itx_memcpy_htod(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int itx_memcpy_dtoh(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int itx_memcpy_dtod(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Pinned host memory from the context's caching pool, and a copy in any
+ * direction ordered on the context's stream that does not wait (between
+ * pinned host and device memory, or device and device).  Blocks return to
+ * the pool in stream order: a block freed and handed out again is only
+ * touched by copies enqueued after the ones that used it.  The host must
+ * not read a block a copy writes before the stream has passed it.  Used to
+ * keep inactive DMRG environments in host memory, where the reference
+ * writes them to disk (LocalMPO::setLHlim / setRHlim, mps/localmpo.h:
+ * 620-681). */
+int itx_host_alloc(itx_ctx *ctx, size_t bytes, void **ptr);
+int itx_host_free(itx_ctx *ctx, void *ptr);
+int itx_memcpy_async(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* x[i] = v for i < n (a fill kernel, safe under graph capture). */
 int itx_fill(itx_ctx *ctx, double *x, size_t n, double v);
 

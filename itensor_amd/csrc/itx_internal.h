@@ -96,6 +96,10 @@ struct itx_ctx {
   std::multimap<size_t, void *> free_blocks;
   std::unordered_map<void *, size_t> live;
   size_t cached_bytes = 0;
+  // pinned host blocks (itx_host_alloc): stream-ordered reuse like the
+  // device pool, released at itx_destroy
+  std::multimap<size_t, void *> host_free;
+  std::unordered_map<void *, size_t> host_live;
   // reduction scratch (partials + results)
   double *red = nullptr;
   // pinned host memory: reduction results written by the device

@@ -173,6 +173,8 @@ int itx_destroy(itx_ctx *ctx) {
   for (auto &kv : ctx->free_blocks) hipFree(kv.second);
   for (auto &kv : ctx->live) hipFree(kv.first);
   hipFree(ctx->red);
+  for (auto &kv : ctx->host_free) hipHostFree(kv.second);
+  for (auto &kv : ctx->host_live) hipHostFree(kv.first);
   if (ctx->hres) hipHostFree(ctx->hres);
   if (ctx->stage) hipHostFree(ctx->stage);
   delete ctx;
@@ -218,6 +220,43 @@ int itx_trim(itx_ctx *ctx) {
   for (auto &kv : ctx->free_blocks) hipFree(kv.second);
   ctx->free_blocks.clear();
   ctx->cached_bytes = 0;
+  return ITX_OK;
+}
+
+int itx_host_alloc(itx_ctx *ctx, size_t bytes, void **ptr) {
+  if (!ctx || !ptr) return ITX_ERR_ARG;
+  if (bytes == 0) bytes = 8;
+  bytes = (bytes + 4095) & ~size_t(4095);
+  auto it = ctx->host_free.lower_bound(bytes);
+  if (it != ctx->host_free.end() && it->first <= bytes + bytes / 4 + 65536) {
+    *ptr = it->second;
+    ctx->host_live[*ptr] = it->first;
+    ctx->host_free.erase(it);
+    return ITX_OK;
+  }
+  void *q = nullptr;
+  ITX_CHECK_HIP(ctx, hipHostMalloc(&q, bytes, hipHostMallocDefault));
+  ctx->host_live[q] = bytes;
+  *ptr = q;
+  return ITX_OK;
+}
+
+int itx_host_free(itx_ctx *ctx, void *ptr) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (!ptr) return ITX_OK;
+  auto it = ctx->host_live.find(ptr);
+  if (it == ctx->host_live.end()) return itx::set_err(ctx, ITX_ERR_ARG, "itx_host_free: pointer not from itx_host_alloc");
+  // stream order makes reuse safe: copies into or out of the block are
+  // enqueued on the context's stream, after the copies that used it before
+  ctx->host_free.emplace(it->second, ptr);
+  ctx->host_live.erase(it);
+  return ITX_OK;
+}
+
+int itx_memcpy_async(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (bytes == 0) return ITX_OK;
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
   return ITX_OK;
 }
 

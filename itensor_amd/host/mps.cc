@@ -1,5 +1,6 @@
 // LocalOp::size, LocalMPO::position, svdBond, the DMRG driver and the small
 // host eigensolver used by davidson's Rayleigh-Ritz step.
+#include <algorithm>
 #include <cmath>
 
 #include "mps.h"
@@ -286,6 +287,75 @@ void LocalMPO::fuseOps(IndexSet const &phiInds) {
   if (!lop_.setFused(&W12_, phiInds)) W12_ = ITensor();
 }
 
+// An environment held in pinned host memory: its structure and the block
+// its values were copied into.
+struct LocalMPO::HostEnv {
+  IndexSet inds;
+  SType type = SType::Dense;
+  BlockOffsets offsets;
+  size_t size = 0;
+  void *host = nullptr;
+  ~HostEnv() {
+    if (host && liveCtx()) itx_host_free(liveCtx(), host);
+  }
+};
+
+void LocalMPO::stash(int j) {
+  ITensor &E = PH_[size_t(j)];
+  if (!E) return;
+  auto h = std::make_shared<HostEnv>();
+  h->inds = E.inds();
+  h->type = E.store().type;
+  h->offsets = E.store().offsets;
+  h->size = E.store().size;
+  const size_t bytes = sizeof(double) * std::max<size_t>(h->size, 1);
+  check(itx_host_alloc(ctx(), bytes, &h->host), "itx_host_alloc");
+  if (h->size)
+    check(itx_memcpy_async(ctx(), h->host, E.store().data->p, sizeof(double) * h->size), "itx_memcpy_async");
+  stash_[size_t(j)] = h;
+  E = ITensor();  // the device block returns to the stream-ordered pool
+}
+
+void LocalMPO::restore(int j) {
+  auto &h = stash_[size_t(j)];
+  if (PH_[size_t(j)] || !h) return;
+  auto st = std::make_shared<Storage>();
+  st->type = h->type;
+  st->offsets = h->offsets;
+  st->size = h->size;
+  st->data = devAlloc(h->size);
+  if (h->size) check(itx_memcpy_async(ctx(), st->data->p, h->host, sizeof(double) * h->size), "itx_memcpy_async");
+  PH_[size_t(j)] = ITensor(h->inds, st);
+  h.reset();  // the pinned block returns to its pool in stream order
+}
+
+// setLHlim / setRHlim (localmpo.h:620-668): with doWrite the environment
+// at the old limit leaves device memory and the one at the new limit is
+// brought back
+void LocalMPO::setLHlim(int val) {
+  if (do_write_ && LHlim_ != val && PH_[size_t(LHlim_)]) stash(LHlim_);
+  LHlim_ = val;
+  if (do_write_ && LHlim_ >= 1) restore(LHlim_);
+}
+
+void LocalMPO::setRHlim(int val) {
+  if (do_write_ && RHlim_ != val && PH_[size_t(RHlim_)]) stash(RHlim_);
+  RHlim_ = val;
+  if (do_write_ && RHlim_ <= int(H_->size())) restore(RHlim_);
+}
+
+int LocalMPO::residentEnvs() const {
+  int n = 0;
+  for (auto const &E : PH_) n += bool(E) ? 1 : 0;
+  return n;
+}
+
+int LocalMPO::hostEnvs() const {
+  int n = 0;
+  for (auto const &h : stash_) n += h ? 1 : 0;
+  return n;
+}
+
 void LocalMPO::position(int b, MPS const &psi) {
   HostTimer ht(HT_ENV);
   auto t0 = std::chrono::steady_clock::now();
@@ -296,7 +366,8 @@ void LocalMPO::position(int b, MPS const &psi) {
     E *= H[size_t(ll)];
     E *= dag(prime(psi(ll + 1)));
     PH_[size_t(ll + 1)] = E;
-    LHlim_ = ll + 1;
+    stash_[size_t(ll + 1)].reset();  // superseded
+    setLHlim(ll + 1);
   }
   while (RHlim_ > b + 2) {
     const int rl = RHlim_;
@@ -304,10 +375,11 @@ void LocalMPO::position(int b, MPS const &psi) {
     E *= H[size_t(rl - 2)];
     E *= dag(prime(psi(rl - 1)));
     PH_[size_t(rl - 1)] = E;
-    RHlim_ = rl - 1;
+    stash_[size_t(rl - 1)].reset();
+    setRHlim(rl - 1);
   }
-  LHlim_ = b - 1;
-  RHlim_ = b + 2;
+  setLHlim(b - 1);
+  setRHlim(b + 2);
   lop_.update(&H[size_t(b - 1)], &H[size_t(b)], &PH_[size_t(b - 1)], &PH_[size_t(b + 2)]);
   b_ = b;
   env_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -423,6 +495,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   PH.setCoverOutput(fullBlocks);
   const bool syncTiming = args.getBool("SyncTiming", false);
   const long finalFP64 = args.getInt("FinalFP64Sweeps", 0);
+  const long writeDim = args.getInt("WriteDim", -1);
   Real energy = NAN;
   for (size_t sw = 0; sw < sweeps.size(); ++sw) {
     auto const &S = sweeps[sw];
@@ -436,6 +509,9 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
     // precision schedule: the last FinalFP64Sweeps sweeps run the H*psi
     // contractions in FP64 whatever HpsiPrecision says
     if (long(sweeps.size() - sw) <= finalFP64) PH.setPrecision(ITX_PREC_F64);
+    // dmrg.h:390-402: past WriteDim only the environments in use stay on
+    // the device
+    if (!PH.doWrite() && writeDim >= 0 && S.maxdim >= writeDim) PH.doWrite(true);
     for (int b = 1, ha = 1; ha <= 2; sweepnext(b, ha, N)) {
       using clk = std::chrono::steady_clock;
       auto secs = [](clk::time_point a, clk::time_point b2) { return std::chrono::duration<double>(b2 - a).count(); };
@@ -465,7 +541,8 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
       auto t3 = mark();
       if (obs) {
         BondRecord r{int(sw + 1), b, ha, energy, spec.truncerr, commonIndex(psi(b), psi(b + 1)).dim(), st.products,
-                     secs(t0, t3), secs(t0, t1), secs(t1, t2), secs(t2, t3), hf};
+                     secs(t0, t3), secs(t0, t1), secs(t1, t2), secs(t2, t3), hf, PH.residentEnvs(),
+                     PH.hostEnvs()};
         obs(r);
       }
     }

@@ -122,7 +122,8 @@ class LocalOp {
 
 class LocalMPO {
  public:
-  explicit LocalMPO(MPO const &H) : H_(&H), PH_(H.size() + 2), LHlim_(0), RHlim_(int(H.size()) + 1) {}
+  explicit LocalMPO(MPO const &H)
+      : H_(&H), PH_(H.size() + 2), stash_(H.size() + 2), LHlim_(0), RHlim_(int(H.size()) + 1) {}
   // position (localmpo.h:443-470) with makeL / makeR (:542-618)
   void position(int b, MPS const &psi);
   void product(ITensor const &phi, ITensor &phip) const { lop_.product(phi, phip); }
@@ -141,10 +142,27 @@ class LocalMPO {
   // pre-contract the two MPO tensors of the current bond for phi's index set
   void fuseOps(IndexSet const &phiInds);
   void setFusion(bool on) { fuse_ = on; }
+  // doWrite (localmpo.h:620-681): only the two environments in use stay in
+  // device memory; the others move to pinned host memory when the limits
+  // pass them and come back when a limit returns (the reference writes
+  // them to files instead).  Copies are ordered on the stream and never
+  // waited for.
+  bool doWrite() const { return do_write_; }
+  void doWrite(bool on) { do_write_ = on; }
+  // environments now held in device / in host memory (diagnostics)
+  int residentEnvs() const;
+  int hostEnvs() const;
 
  private:
+  struct HostEnv;
+  void setLHlim(int val);
+  void setRHlim(int val);
+  void stash(int j);
+  void restore(int j);
   MPO const *H_;
   std::vector<ITensor> PH_;
+  std::vector<std::shared_ptr<HostEnv>> stash_;
+  bool do_write_ = false;
   int LHlim_, RHlim_;
   LocalOp lop_;
   ITensor W12_;
@@ -181,6 +199,8 @@ struct BondRecord {
   // algorithmic flops of the bond's H*psi products by the arithmetic form
   // their kernels computed in (ITX_PREC_* index; zero in Eager mode)
   std::array<double, 3> hpsi_flops{};
+  // environments held in device and in host memory after the step
+  int envs_device = 0, envs_host = 0;
 };
 
 // MPS::svdBond (mps/mps_impl.h:21-99).  With Args "UseSVD", or with no

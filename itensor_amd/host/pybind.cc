@@ -79,6 +79,8 @@ py::dict recordDict(BondRecord const &r) {
   d["t_davidson"] = r.t_davidson;
   d["t_svd"] = r.t_svd;
   d["hpsi_flops"] = std::vector<double>(r.hpsi_flops.begin(), r.hpsi_flops.end());
+  d["envs_device"] = r.envs_device;
+  d["envs_host"] = r.envs_host;
   return d;
 }
 

@@ -47,6 +47,9 @@ _PROTOS = {
     "itx_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "itx_memcpy_dtod": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "itx_fill": (C.c_int, [_vp, _vp, C.c_size_t, C.c_double]),
+    "itx_host_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "itx_host_free": (C.c_int, [_vp, _vp]),
+    "itx_memcpy_async": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "itx_contract_dense": (C.c_int, [_vp, _vp, C.c_int, _i64p, _i32p, _vp, C.c_int, _i64p, _i32p,
                                      _vp, C.c_int, _i64p, _i32p, C.c_double, C.c_double, C.c_int]),
     "itx_contract_qn": (C.c_int, [_vp, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p,

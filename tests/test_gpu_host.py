@@ -579,3 +579,27 @@ def test_position_orthogonalises_without_changing_the_state(H):
         lk = [I for I in T.inds if "Link" in I.tags and any(I.id == K.id for K in oten(tens[j - 1]).inds)][0]
         G = T * T.prime(1, lk).dag()
         assert np.allclose(G.to_dense_array(), np.eye(lk.dim), atol=1e-12)
+
+
+@pytest.mark.parametrize("qn", [False, True])
+def test_environment_offload_write_dim(H, qn):
+    """Args WriteDim (dmrg.h:390-402, localmpo.h:620-681): from the first
+    sweep whose maxdim reaches it, only the two environments in use stay in
+    device memory and the rest sit in pinned host memory; the copies are
+    exact, so every bond energy equals that of the run without offload."""
+    N = 14
+    sites = OD.spin_half_sites(N, qn)
+    Hm = OD.heisenberg_mpo(sites)
+    psi0 = OD.product_mps(sites, [i % 2 for i in range(N)]) if qn else OD.random_mps(sites, 4, seed=2)
+    sw = [{"MaxDim": m, "Cutoff": 1e-12, "Niter": 2} for m in (8, 16, 32, 32)]
+    runs = []
+    for wd in (None, 16):
+        args = {} if wd is None else {"WriteDim": wd}
+        runs.append(H.dmrg([cten(w) for w in Hm], [cten(a) for a in psi0.A], sw, args))
+    (E0, r0, _, _), (E1, r1, _, _) = runs
+    assert [r["energy"] for r in r0] == [r["energy"] for r in r1]
+    assert [r["dim"] for r in r0] == [r["dim"] for r in r1]
+    on = [r for r in r1 if r["sweep"] >= 2]
+    assert max(r["envs_device"] for r in on) <= 2
+    assert min(r["envs_host"] for r in on[2 * N:]) >= N - 4
+    assert max(r["envs_device"] for r in r0) > 2
