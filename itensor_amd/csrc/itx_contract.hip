@@ -72,27 +72,33 @@ struct MF<float> {
 // (out-of-range elements read element 0 of the block and are zeroed), which
 // keeps the load stream free of exec-mask branches.
 // Offsets stay 64-bit: 32-bit offsets measured 1.6x slower on gfx950.
-template <typename T, int BR, bool KC, int NTH, int BK = 16>
+// With V = 2 (k-contiguous operands whose k runs come in aligned pairs,
+// checked when the plan is built) a lane loads two consecutive k of one
+// row as one 16-byte load and fetches one table entry per pair.
+template <typename T, int BR, bool KC, int NTH, int BK = 16, int V = 1>
 struct TileLoader {
   static constexpr int PER = BR * BK / NTH;  // elements per thread
   static_assert(PER >= 1, "tile too small for the thread count");
+  static_assert(V == 1 || (KC && PER % V == 0 && BK % V == 0), "vector loads need a k-contiguous operand");
+  static constexpr int NV = PER / V;         // loads per thread
+  static constexpr int LK = BK / V;          // lanes along k (KC)
   static constexpr int LD = BR + MF<T>::template pad<BR>();
   static constexpr int NK = KC ? 1 : PER;  // k offsets per thread and stage
   double r[PER];
-  bool rok[PER];  // element in range (applied at store, after the MFMAs)
-  int64_t roff[KC ? PER : 1];
-  bool rv[KC ? PER : 1];
+  bool rok[NV];  // element in range (applied at store, after the MFMAs)
+  int64_t roff[KC ? NV : 1];
+  bool rv[KC ? NV : 1];
   int64_t ko[NK];
   bool kv[NK];
   int lr, lk;  // thread's base row / base k
 
   __device__ __forceinline__ void init(int tid, int row0, int R, const int64_t *__restrict__ tabR) {
     if constexpr (KC) {
-      lk = tid % BK;
-      lr = tid / BK;
+      lk = (tid % LK) * V;
+      lr = tid / LK;
 #pragma unroll
-      for (int p = 0; p < PER; ++p) {
-        int row = row0 + lr + (NTH / BK) * p;
+      for (int p = 0; p < NV; ++p) {
+        int row = row0 + lr + (NTH / LK) * p;
         rv[p] = row < R;
         roff[p] = tabR[rv[p] ? row : 0];
       }
@@ -116,19 +122,29 @@ struct TileLoader {
   // data loads of the stage whose offsets fetch_k last read
   __device__ __forceinline__ void load(const double *__restrict__ base) {
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
+    for (int p = 0; p < NV; ++p) {
       const bool ok = KC ? (kv[0] && rv[p]) : (kv[p] && rv[0]);
       const int64_t off = (KC ? roff[p] + ko[0] : roff[0] + ko[p]);
-      r[p] = base[ok ? off : 0];
+      if constexpr (V == 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(base + (ok ? off : 0));
+        r[2 * p] = v.x;
+        r[2 * p + 1] = v.y;
+      } else {
+        r[p] = base[ok ? off : 0];
+      }
       rok[p] = ok;
     }
   }
   __device__ __forceinline__ void store(T *__restrict__ s) const {
     if constexpr (KC) {
 #pragma unroll
-      for (int p = 0; p < PER; ++p) {
-        int row = lr + (NTH / BK) * p;
-        s[lk * LD + (row ^ MF<T>::sw(lk))] = rok[p] ? T(r[p]) : T(0);
+      for (int p = 0; p < NV; ++p) {
+        int row = lr + (NTH / LK) * p;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int k = lk + e;
+          s[k * LD + (row ^ MF<T>::sw(k))] = rok[p] ? T(r[V * p + e]) : T(0);
+        }
       }
     } else {
 #pragma unroll
@@ -142,15 +158,15 @@ struct TileLoader {
 
 // One workgroup = GM x GN waves computing a BM x BN tile of C; each wave
 // owns a (BM/GM) x (BN/GN) sub-tile of 16x16 MFMA fragments.
-template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC, int BK>
+template <typename T, int BM, int BN, int GM, int GN, bool AKC, bool BKC, int BK, int V>
 __global__ __launch_bounds__(64 * GM * GN) void k_gemm(const double *__restrict__ A, const double *__restrict__ B,
                                                        double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                        const WorkUnit *__restrict__ units, int nunits,
                                                        const int64_t *__restrict__ tab, double alpha, double beta) {
   constexpr int NTH = 64 * GM * GN;
   using v4 = typename MF<T>::v4;
-  using LA = TileLoader<T, BM, AKC, NTH, BK>;
-  using LB = TileLoader<T, BN, BKC, NTH, BK>;
+  using LA = TileLoader<T, BM, AKC, NTH, BK, V>;
+  using LB = TileLoader<T, BN, BKC, NTH, BK, V>;
   constexpr int WM = BM / GM, WN = BN / GN, FM = WM / 16, FN = WN / 16;
   static_assert(FM >= 1 && FN >= 1, "wave tile below one MFMA fragment");
   __shared__ T As[2][BK * LA::LD];
@@ -296,26 +312,33 @@ __device__ __forceinline__ void split_bf16(double x, unsigned short &hi, unsigne
   lo = (unsigned short)bf16_rne(__float_as_uint(rem));
 }
 
-template <int BR, bool KC, int NTH>
+// With V = 4 (k-contiguous operands whose k runs come in aligned quads,
+// checked when the plan is built) a lane loads four consecutive k of one
+// row as two 16-byte loads and writes their hi and lo terms to LDS as one
+// 8-byte store each.
+template <int BR, bool KC, int NTH, int V = 1>
 struct TileLoader3 {
   static constexpr int PER = BR * BK3 / NTH;
   static_assert(PER >= 1, "tile too small for the thread count");
+  static_assert(V == 1 || (KC && V == 4 && PER % V == 0), "vector loads need a k-contiguous operand");
+  static constexpr int NV = PER / V;
+  static constexpr int LK = BK3 / V;
   static constexpr int NK = KC ? 1 : PER;
   double r[PER];
-  bool rok[PER];
-  int64_t roff[KC ? PER : 1];
-  bool rv[KC ? PER : 1];
+  bool rok[NV];
+  int64_t roff[KC ? NV : 1];
+  bool rv[KC ? NV : 1];
   int64_t ko[NK];
   bool kv[NK];
   int lr, lk;
 
   __device__ __forceinline__ void init(int tid, int row0, int R, const int64_t *__restrict__ tabR) {
     if constexpr (KC) {
-      lk = tid % BK3;
-      lr = tid / BK3;
+      lk = (tid % LK) * V;
+      lr = tid / LK;
 #pragma unroll
-      for (int p = 0; p < PER; ++p) {
-        const int row = row0 + lr + (NTH / BK3) * p;
+      for (int p = 0; p < NV; ++p) {
+        const int row = row0 + lr + (NTH / LK) * p;
         rv[p] = row < R;
         roff[p] = tabR[rv[p] ? row : 0];
       }
@@ -337,34 +360,59 @@ struct TileLoader3 {
   }
   __device__ __forceinline__ void load(const double *__restrict__ base) {
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
+    for (int p = 0; p < NV; ++p) {
       const bool ok = KC ? (kv[0] && rv[p]) : (kv[p] && rv[0]);
       const int64_t off = (KC ? roff[p] + ko[0] : roff[0] + ko[p]);
-      r[p] = base[ok ? off : 0];
+      if constexpr (V == 4) {
+        const double2 *q = reinterpret_cast<const double2 *>(base + (ok ? off : 0));
+        const double2 v0 = q[0], v1 = q[1];
+        r[4 * p] = v0.x;
+        r[4 * p + 1] = v0.y;
+        r[4 * p + 2] = v1.x;
+        r[4 * p + 3] = v1.y;
+      } else {
+        r[p] = base[ok ? off : 0];
+      }
       rok[p] = ok;
     }
   }
   __device__ __forceinline__ void store(unsigned short *__restrict__ sh, unsigned short *__restrict__ sl) const {
+    if constexpr (V == 4) {
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int idx = KC ? (lr + (NTH / BK3) * p) * LDK3 + lk : lr * LDK3 + lk + (NTH / BR) * p;
-      unsigned short h = 0, l = 0;
-      if (rok[p]) split_bf16(r[p], h, l);
-      sh[idx] = h;
-      sl[idx] = l;
+      for (int p = 0; p < NV; ++p) {
+        const int idx = (lr + (NTH / LK) * p) * LDK3 + lk;
+        unsigned short h[4] = {0, 0, 0, 0}, l[4] = {0, 0, 0, 0};
+        if (rok[p]) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) split_bf16(r[4 * p + e], h[e], l[e]);
+        }
+        *reinterpret_cast<uint2 *>(&sh[idx]) =
+            uint2{uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16)};
+        *reinterpret_cast<uint2 *>(&sl[idx]) =
+            uint2{uint32_t(l[0]) | (uint32_t(l[1]) << 16), uint32_t(l[2]) | (uint32_t(l[3]) << 16)};
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        const int idx = KC ? (lr + (NTH / BK3) * p) * LDK3 + lk : lr * LDK3 + lk + (NTH / BR) * p;
+        unsigned short h = 0, l = 0;
+        if (rok[p]) split_bf16(r[p], h, l);
+        sh[idx] = h;
+        sl[idx] = l;
+      }
     }
   }
 };
 
-template <int BM, int BN, int GM, int GN, bool AKC, bool BKC>
+template <int BM, int BN, int GM, int GN, bool AKC, bool BKC, int V>
 __global__ __launch_bounds__(64 * GM * GN) void k_gemm3(const double *__restrict__ A, const double *__restrict__ B,
                                                         double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                         const WorkUnit *__restrict__ units, int nunits,
                                                         const int64_t *__restrict__ tab, double alpha, double beta) {
   constexpr int NTH = 64 * GM * GN;
   typedef float v4 __attribute__((ext_vector_type(4)));
-  using LA = TileLoader3<BM, AKC, NTH>;
-  using LB = TileLoader3<BN, BKC, NTH>;
+  using LA = TileLoader3<BM, AKC, NTH, V>;
+  using LB = TileLoader3<BN, BKC, NTH, V>;
   constexpr int WM = BM / GM, WN = BN / GN, FM = WM / 16, FN = WN / 16;
   static_assert(FM >= 1 && FN >= 1, "wave tile below one MFMA fragment");
   __shared__ __align__(16) unsigned short Ah[2][BM * LDK3], Al[2][BM * LDK3];
@@ -1040,61 +1088,65 @@ constexpr int kSkinnyOcc = 4;
 
 // Kernel instantiation of a tile shape, operand layout and form.  All
 // kernels of the engine share one argument list (see KernelArgs).
+// vec: both operands k-contiguous with aligned runs of 2 (FP64 / FP32
+// loaders) or 4 (BF16x3 loader) consecutive k -- the vector-load variants
 template <typename T, int BM, int BN, int GM, int GN, int BKS = BK>
-const void *gemm_fn(bool akc, bool bkc) {
+const void *gemm_fn(bool akc, bool bkc, bool vec) {
+  if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true, BKS, 2>);
   if (akc)
-    return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true, BKS>)
-               : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false, BKS>);
-  return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true, BKS>)
-             : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false, BKS>);
+    return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, true, BKS, 1>)
+               : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, true, false, BKS, 1>);
+  return bkc ? reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, true, BKS, 1>)
+             : reinterpret_cast<const void *>(&k_gemm<T, BM, BN, GM, GN, false, false, BKS, 1>);
 }
 
 template <int BM, int BN, int GM, int GN>
-const void *gemm3_fn(bool akc, bool bkc) {
+const void *gemm3_fn(bool akc, bool bkc, bool vec) {
+  if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true, 4>);
   if (akc)
-    return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true>)
-               : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, false>);
-  return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, true>)
-             : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, false>);
+    return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true, 1>)
+               : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, false, 1>);
+  return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, true, 1>)
+             : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, false, 1>);
 }
 
-const void *gemm3_fn_tile(int tile, bool akc, bool bkc) {
+const void *gemm3_fn_tile(int tile, bool akc, bool bkc, bool vec) {
   switch (tile) {
-    case 0: return gemm3_fn<128, 128, 2, 2>(akc, bkc);
-    case 1: return gemm3_fn<128, 64, 2, 2>(akc, bkc);
-    case 2: return gemm3_fn<64, 128, 2, 2>(akc, bkc);
-    case 3: return gemm3_fn<64, 64, 2, 2>(akc, bkc);
-    case 6: return gemm3_fn<128, 128, 2, 4>(akc, bkc);
-    case 7: return gemm3_fn<256, 64, 4, 2>(akc, bkc);
-    case 8: return gemm3_fn<64, 256, 2, 4>(akc, bkc);
-    case 9: return gemm3_fn<256, 128, 4, 2>(akc, bkc);
-    case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc);
-    case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc);
-    case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc);
-    default: return gemm3_fn<32, 32, 2, 2>(akc, bkc);
+    case 0: return gemm3_fn<128, 128, 2, 2>(akc, bkc, vec);
+    case 1: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec);
+    case 2: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec);
+    case 3: return gemm3_fn<64, 64, 2, 2>(akc, bkc, vec);
+    case 6: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec);
+    case 7: return gemm3_fn<256, 64, 4, 2>(akc, bkc, vec);
+    case 8: return gemm3_fn<64, 256, 2, 4>(akc, bkc, vec);
+    case 9: return gemm3_fn<256, 128, 4, 2>(akc, bkc, vec);
+    case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec);
+    case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec);
+    case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec);
+    default: return gemm3_fn<32, 32, 2, 2>(akc, bkc, vec);
   }
 }
 
 template <typename T>
-const void *gemm_fn_tile(int tile, bool akc, bool bkc) {
+const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
   switch (tile) {
-    case 0: return gemm_fn<T, 128, 128, 2, 2>(akc, bkc);
-    case 1: return gemm_fn<T, 128, 64, 2, 2>(akc, bkc);
-    case 2: return gemm_fn<T, 64, 128, 2, 2>(akc, bkc);
-    case 3: return gemm_fn<T, 64, 64, 2, 2>(akc, bkc);
-    case 6: return gemm_fn<T, 128, 128, 2, 4>(akc, bkc);
-    case 7: return gemm_fn<T, 256, 64, 4, 2>(akc, bkc);
-    case 8: return gemm_fn<T, 64, 256, 2, 4>(akc, bkc);
-    case 9: return gemm_fn<T, 256, 128, 4, 2>(akc, bkc);
-    case 11: return gemm_fn<T, 128, 128, 2, 4, 32>(akc, bkc);
-    case 12: return gemm_fn<T, 128, 64, 2, 2, 32>(akc, bkc);
-    case 13: return gemm_fn<T, 64, 128, 2, 2, 32>(akc, bkc);
-    default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc);
+    case 0: return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, vec);
+    case 1: return gemm_fn<T, 128, 64, 2, 2>(akc, bkc, vec);
+    case 2: return gemm_fn<T, 64, 128, 2, 2>(akc, bkc, vec);
+    case 3: return gemm_fn<T, 64, 64, 2, 2>(akc, bkc, vec);
+    case 6: return gemm_fn<T, 128, 128, 2, 4>(akc, bkc, vec);
+    case 7: return gemm_fn<T, 256, 64, 4, 2>(akc, bkc, vec);
+    case 8: return gemm_fn<T, 64, 256, 2, 4>(akc, bkc, vec);
+    case 9: return gemm_fn<T, 256, 128, 4, 2>(akc, bkc, vec);
+    case 11: return gemm_fn<T, 128, 128, 2, 4, 32>(akc, bkc, vec);
+    case 12: return gemm_fn<T, 128, 64, 2, 2, 32>(akc, bkc, vec);
+    case 13: return gemm_fn<T, 64, 128, 2, 2, 32>(akc, bkc, vec);
+    default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc, vec);
   }
 }
 
 // Entry point and workgroup size of a plan's kernel.
-const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block) {
+const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int vec = 0) {
   if (tile == kSkinny) {  // HBM-bound: FP64 arithmetic in every form
     *block = SK_ROWS;
     return reinterpret_cast<const void *>(&k_skinny);
@@ -1119,8 +1171,10 @@ const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block) {
                : reinterpret_cast<const void *>(&k_small<false, false>);
   }
   *block = 64 * kTiles[tile].gm * kTiles[tile].gn;
-  if (prec == ITX_PREC_BF16X3) return gemm3_fn_tile(tile, akc, bkc);
-  return prec == ITX_PREC_F64 ? gemm_fn_tile<double>(tile, akc, bkc) : gemm_fn_tile<float>(tile, akc, bkc);
+  // vec: bit 0 = runs of 2 usable (FP64 / FP32), bit 1 = runs of 4 (BF16x3)
+  if (prec == ITX_PREC_BF16X3) return gemm3_fn_tile(tile, akc, bkc, (vec & 2) != 0);
+  return prec == ITX_PREC_F64 ? gemm_fn_tile<double>(tile, akc, bkc, (vec & 1) != 0)
+                              : gemm_fn_tile<float>(tile, akc, bkc, (vec & 1) != 0);
 }
 
 // Resident workgroups per CU of one kernel instantiation (registers and
@@ -1213,6 +1267,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     int64_t tAM, tCM, tBN, tCN, tAK, tBK;
     int64_t am, cm, bn, cn, ak, bk;
     bool akc, bkc, empty;
+    int vec;  // bit 0: k runs of 2 usable by both operands, bit 1: runs of 4
   };
   std::vector<DimInfo> infos;
   std::vector<int64_t> ikeys;  // the dims of each entry (rA + rB per entry)
@@ -1274,8 +1329,33 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     digits_max(dm, fi.am, fi.cm);
     digits_max(dn, fi.bn, fi.cn);
     digits_max(dk, fi.ak, fi.bk);
+    // vector loads: both operands k-contiguous, every aligned group of V
+    // consecutive k one run of V consecutive elements starting at an even
+    // offset, every row offset even (16-byte aligned given an aligned base)
+    fi.vec = 0;
+    if (fi.akc && fi.bkc) {
+      auto runs = [&](int64_t t, int64_t n, int V) {
+        if (n % V) return false;
+        for (int64_t k = 0; k < n; k += V) {
+          if (tab[size_t(t + k)] % 2) return false;
+          for (int j = 1; j < V; ++j)
+            if (tab[size_t(t + k + j)] != tab[size_t(t + k)] + j) return false;
+        }
+        return true;
+      };
+      auto even = [&](int64_t t, int64_t n) {
+        for (int64_t i = 0; i < n; ++i)
+          if (tab[size_t(t + i)] % 2) return false;
+        return true;
+      };
+      if (even(fi.tAM, M) && even(fi.tBN, N)) {
+        if (runs(fi.tAK, K, 2) && runs(fi.tBK, K, 2)) fi.vec |= 1;
+        if (runs(fi.tAK, K, 4) && runs(fi.tBK, K, 4)) fi.vec |= 2;
+      }
+    }
     return ITX_OK;
   };
+  int vecbits = 3;  // AND over the tasks
   for (int t = 0; t < ntasks; ++t) {
     const int64_t *dA = blkA + size_t(t) * rA;
     const int64_t *dB = blkB + size_t(t) * rB;
@@ -1314,6 +1394,8 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
       akc = fi.akc;
       bkc = fi.bkc;
     }
+    vecbits &= fi.vec;
+    if (offs[3 * size_t(t)] % 2 || offs[3 * size_t(t) + 1] % 2) vecbits = 0;
     GemmTask g{};
     g.a_off = offs[3 * size_t(t)];
     g.b_off = offs[3 * size_t(t) + 1];
@@ -1492,7 +1574,9 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->nunits = int(units.size());
     p->flops = flops;
     p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
-    p->fn = kernel_of(best, prec, akc, bkc, &p->block);
+    p->vec = (akc && bkc) ? vecbits : 0;
+    p->fn = kernel_of(best, prec, akc, bkc, &p->block, p->vec);
+    p->fn_scalar = kernel_of(best, prec, akc, bkc, &p->block, 0);
     p->nred = int(red.size());
     if (p->nred) {
       p->fn_red = reinterpret_cast<const void *>(&k_reduce_tiles);
@@ -1630,6 +1714,13 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   return finalize(best, out);
 }
 
+const void *plan_fn(const itx_plan *p, const double *A, const double *B) {
+  // the vector-load kernel reads 16-byte pairs: its operands must be
+  // 16-byte aligned (caching-pool blocks are 256-byte aligned)
+  const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
+  return (p->vec && aligned) ? p->fn : p->fn_scalar;
+}
+
 KernelArgs plan_args(const itx_plan *p, int which, const double *A, const double *B, double *C, double alpha,
                      double beta) {
   if (which == 0)  // the contraction (into the workspace when split)
@@ -1644,7 +1735,7 @@ int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *
   void *args[KernelArgs::kCount];
   KernelArgs ka = plan_args(p, 0, A, B, C, alpha, beta);
   ka.pointers(args);
-  ITX_CHECK_HIP(ctx, hipLaunchKernel(p->fn, dim3(p->grid), dim3(p->block), args, 0, ctx->stream));
+  ITX_CHECK_HIP(ctx, hipLaunchKernel(plan_fn(p, A, B), dim3(p->grid), dim3(p->block), args, 0, ctx->stream));
   if (p->nred) {
     KernelArgs kr = plan_args(p, 1, A, B, C, alpha, beta);
     kr.pointers(args);
@@ -1766,6 +1857,7 @@ int itx_chain_create(itx_ctx *ctx, int nstages, itx_plan *const *plans, itx_chai
   ch->plans.assign(plans, plans + nstages);
   ch->args.resize(2 * size_t(nstages));
   ch->nodes.assign(2 * size_t(nstages), nullptr);
+  ch->funcs.assign(2 * size_t(nstages), nullptr);
   ctx->chains.insert(ch);
   *out = ch;
   return ITX_OK;
@@ -1785,10 +1877,28 @@ int itx_chain_launch(itx_ctx *ctx, itx_chain *ch, const double *const *A, const 
     const itx_plan *p = ch->plans[size_t(x / 2)];
     return p->nunits > 0 && (x % 2 == 0 || p->nred > 0);
   };
-  auto params_of = [&](int x, void **argv) {
+  auto fn_of = [&](int x) {
     const itx_plan *p = ch->plans[size_t(x / 2)];
+    return x % 2 ? p->fn_red : itx::plan_fn(p, A[x / 2], B[x / 2]);
+  };
+  if (ch->exec) {
+    // a changed operand alignment changes a stage's kernel: rebuild
+    bool same = true;
+    for (int x = 0; x < 2 * n && same; ++x)
+      if (ch->nodes[size_t(x)] && fn_of(x) != ch->funcs[size_t(x)]) same = false;
+    if (!same) {
+      hipGraphExecDestroy(ch->exec);
+      hipGraphDestroy(ch->graph);
+      ch->exec = nullptr;
+      ch->graph = nullptr;
+      ch->nodes.assign(ch->nodes.size(), nullptr);
+    }
+  }
+  auto params_of = [&](int x, void **argv) {
     hipKernelNodeParams kp{};
-    kp.func = const_cast<void *>(x % 2 ? p->fn_red : p->fn);
+    const itx_plan *p = ch->plans[size_t(x / 2)];
+    ch->funcs[size_t(x)] = fn_of(x);
+    kp.func = const_cast<void *>(ch->funcs[size_t(x)]);
     kp.gridDim = dim3(unsigned(x % 2 ? p->grid_red : p->grid));
     kp.blockDim = dim3(unsigned(x % 2 ? itx::SM_T_RED : p->block));
     kp.sharedMemBytes = 0;

@@ -67,6 +67,8 @@ struct itx_plan {
   const itx::WorkUnit *d_units = nullptr;
   const int64_t *d_tab = nullptr;
   const void *fn = nullptr;  // kernel entry point and workgroup size
+  const void *fn_scalar = nullptr;  // the same tile without vector loads
+  int vec = 0;  // fn uses 16-byte operand loads (operands must be 16-byte aligned)
   int block = 0;
   // split plans: partial tiles into ws, then a reduction pass over
   // nred reduction units (d_red) writes C
@@ -83,6 +85,7 @@ struct itx_chain {
   std::vector<const itx_plan *> plans;
   std::vector<itx::KernelArgs> args;  // arguments the graph nodes hold now
   std::vector<hipGraphNode_t> nodes;  // 2 per stage; null where no launch
+  std::vector<const void *> funcs;    // the kernel each node runs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -146,6 +149,8 @@ int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *
                 double alpha, double beta);
 // Arguments of launch `which` of a plan (0: the contraction, 1: the
 // reduction pass of a split plan) for operands A, B, C.
+// the kernel a plan launches for operands A, B (vector loads when aligned)
+const void *plan_fn(const itx_plan *p, const double *A, const double *B);
 KernelArgs plan_args(const itx_plan *p, int which, const double *A, const double *B, double *C, double alpha,
                      double beta);
 constexpr int SM_T_RED = 256;  // workgroup size of the reduction pass
