@@ -411,7 +411,10 @@ Spectrum svdOrd2(ITensor const &A, Index const &ui, Index const &vi, ITensor &U,
   };
   U = build(Uis, true);
   V = build(Vis, false);
-  auto [doffs, dsize] = getBlockOffsets(Dis, QN());
+  // D holds A's divergence (the reference's QDiagReal over (L, R) has the
+  // diagonal blocks whatever the flux; with U and V flux-free they are
+  // exactly the blocks of flux div(A))
+  auto [doffs, dsize] = getBlockOffsets(Dis, div(A));
   std::vector<double> dh(size_t(dsize), 0.0);
   for (int b = 0, nk = 0; b < nb; ++b) {
     if (keep[size_t(b)] == 0) continue;

@@ -126,6 +126,7 @@ bool LocalOp::setFused(ITensor const *W12, IndexSet const &phiInds) {
 struct LocalOp::Program {
   IndexSet phiInds;
   bool qn = false;
+  int prec = ITX_PREC_F64;
   BlockOffsets phiOffs;
   std::vector<ITensor const *> ops;
   std::vector<std::unique_ptr<ContractPlan>> stages;
@@ -157,7 +158,8 @@ std::vector<ITensor const *> LocalOp::chainOps() const {
 LocalOp::Program *LocalOp::program(ITensor const &phi) const {
   if (mode_ == HpsiMode::Eager || !phi || phi.isCombiner() || phi.order() == 0) return nullptr;
   for (auto it = progs_.rbegin(); it != progs_.rend(); ++it)
-    if (ContractPlan::sameStructure(phi, (*it)->phiInds, (*it)->qn, (*it)->phiOffs)) return it->get();
+    if ((*it)->prec == prec_ && ContractPlan::sameStructure(phi, (*it)->phiInds, (*it)->qn, (*it)->phiOffs))
+      return it->get();
   auto ops = chainOps();
   for (auto *o : ops)
     if (!o || !*o || o->isCombiner() || o->order() == 0 || o->isQN() != phi.isQN()) return nullptr;
@@ -165,6 +167,7 @@ LocalOp::Program *LocalOp::program(ITensor const &phi) const {
   auto prg = std::make_shared<Program>();
   prg->phiInds = phi.inds();
   prg->qn = phi.isQN();
+  prg->prec = prec_;
   if (prg->qn) prg->phiOffs = phi.store().offsets;
   prg->ops = ops;
   // structure-only stand-in for each stage's left operand
@@ -496,6 +499,7 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
   const bool syncTiming = args.getBool("SyncTiming", false);
   const long finalFP64 = args.getInt("FinalFP64Sweeps", 0);
   const long writeDim = args.getInt("WriteDim", -1);
+  const bool energyFP64 = args.getBool("EnergyFP64", true);
   Real energy = NAN;
   for (size_t sw = 0; sw < sweeps.size(); ++sw) {
     auto const &S = sweeps[sw];
@@ -531,18 +535,36 @@ Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args_in,
       PH.fuseOps(phi.inds());
       DavidsonStats st;
       energy = davidson(PH, phi, args, &st);
+      const double ritz = energy;
       std::array<double, 3> hf{};
       if (obs) {
         hf = PH.flopsByForm(phi);
         for (auto &x : hf) x *= st.products;
       }
+      // Reduced-precision products leave an error of the order of the
+      // form's rounding times |H| in the Ritz value, while the Ritz vector
+      // is accurate to second order.  The energy a sweep reports is then
+      // the FP64 Rayleigh quotient of its last Ritz vector: one FP64
+      // product per sweep.
+      double e64 = NAN;
+      const bool lastStep = ha == 2 && b == 1;
+      if (energyFP64 && lastStep && PH.precision() != ITX_PREC_F64) {
+        const int p = PH.precision();
+        PH.setPrecision(ITX_PREC_F64);
+        ITensor Hphi;
+        PH.product(phi, Hphi);
+        PH.setPrecision(p);
+        e64 = dotDag(phi, Hphi) / dotDag(phi, phi);
+        energy = e64;
+        if (obs) hf[0] += PH.flops(phi);  // the FP64 product's flops
+      }
       auto t2 = mark();
       auto spec = svdBond(psi, b, phi, ha == 1 ? Fromleft : Fromright, args, &PH);
       auto t3 = mark();
       if (obs) {
-        BondRecord r{int(sw + 1), b, ha, energy, spec.truncerr, commonIndex(psi(b), psi(b + 1)).dim(), st.products,
+        BondRecord r{int(sw + 1), b, ha, ritz, spec.truncerr, commonIndex(psi(b), psi(b + 1)).dim(), st.products,
                      secs(t0, t3), secs(t0, t1), secs(t1, t2), secs(t2, t3), hf, PH.residentEnvs(),
-                     PH.hostEnvs()};
+                     PH.hostEnvs(), e64};
         obs(r);
       }
     }

@@ -75,10 +75,10 @@ class LocalOp {
   // contractions run in the precision form `prec` (ITX_PREC_*)
   void product(ITensor const &phi, ITensor &phip) const;
   long size() const;  // localop.h:488-530
-  void setPrecision(int p) {
-    if (p != prec_) progs_.clear();
-    prec_ = p;
-  }
+  // programs are kept per precision form, so switching forms (e.g. one
+  // FP64 product among reduced-precision ones) rebuilds nothing
+  void setPrecision(int p) { prec_ = p; }
+  int precision() const { return prec_; }
   void setMode(HpsiMode m) {
     if (m != mode_) progs_.clear();
     mode_ = m;
@@ -131,6 +131,7 @@ class LocalMPO {
   ITensor const &L() const { return PH_.at(size_t(LHlim_)); }
   ITensor const &R() const { return PH_.at(size_t(RHlim_)); }
   void setPrecision(int p) { lop_.setPrecision(p); }
+  int precision() const { return lop_.precision(); }
   void setMode(HpsiMode m) { lop_.setMode(m); }
   void setCoverOutput(bool on) { lop_.setCoverOutput(on); }
   double flops(ITensor const &phi) const { return lop_.flops(phi); }
@@ -201,6 +202,10 @@ struct BondRecord {
   std::array<double, 3> hpsi_flops{};
   // environments held in device and in host memory after the step
   int envs_device = 0, envs_host = 0;
+  // the FP64 Rayleigh quotient <phi|H|phi> of the Ritz vector, computed
+  // with one FP64 product when the step's products ran in a reduced form
+  // and it ends a sweep (NaN otherwise)
+  double energy_fp64 = NAN;
 };
 
 // MPS::svdBond (mps/mps_impl.h:21-99).  With Args "UseSVD", or with no
@@ -222,6 +227,11 @@ void position(MPS &psi, int i, Args const &args = Args());
 
 // dmrg (mps/dmrg.h:51-76, 336-467).  obs (optional) receives one record
 // per bond step; sweep_seconds (optional) the wall time of each sweep.
+// Precision Args (no counterpart in the reference): "HpsiPrecision"
+// (ITX_PREC_*) runs davidson's H*psi products in a reduced form;
+// "FinalFP64Sweeps" n runs the last n sweeps in FP64; "EnergyFP64"
+// (default true) makes the energy of a sweep run in a reduced form the FP64
+// Rayleigh quotient of its last Ritz vector (one FP64 product per sweep).
 Real dmrg(MPS &psi, MPO const &H, Sweeps const &sweeps, Args const &args,
           std::function<void(BondRecord const &)> obs = nullptr, std::vector<double> *sweep_seconds = nullptr);
 

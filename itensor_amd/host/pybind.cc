@@ -71,6 +71,7 @@ py::dict recordDict(BondRecord const &r) {
   d["b"] = r.b;
   d["ha"] = r.ha;
   d["energy"] = r.energy;
+  d["energy_fp64"] = r.energy_fp64;
   d["truncerr"] = r.truncerr;
   d["dim"] = r.dim;
   d["nprod"] = r.products;

@@ -526,7 +526,9 @@ def svd(AA: ITensor, uinds, args: dict):
         R = Index(Riq, rtags, dir=vi.dir)
         U = qdense([ui, L.dag()], QN())
         V = qdense([vi, R.dag()], QN())
-        D = qdense([L, R], QN())
+        # D carries AA's divergence (the reference's QDiagReal keeps the
+        # diagonal blocks whatever the flux; U and V are flux-free)
+        D = qdense([L, R], AAc.div())
         n = 0
         for (b1, b2, M), d, UU, VV, k in zip(blocks, dvecs, umats, vmats, kept):
             if not k:

@@ -213,11 +213,12 @@ def test_chain_graph_replay_matches_direct(ctx, dev):
             p.close()
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
 def test_small_block_kernel_general_shapes(ctx, dev, monkeypatch, prec):
-    """The small-block VALU kernel forced onto every case shape (K above one
+    """The small-block kernels forced onto every case shape (K above one
     64-wide chunk, N above one 32-wide unit, rank-0 results, permuted
-    operands), with alpha and beta: FP64 arithmetic in every form."""
+    operands), with alpha and beta: FP64 on the VALU, FP32 and BF16x3 on
+    the matrix cores, each to its form's tolerance."""
     monkeypatch.setenv("ITX_FORCE_TILE", "10")
     rng = np.random.default_rng(55)
     for case, (dA, labA, dB, labB, labC) in enumerate(CASES):
@@ -231,9 +232,15 @@ def test_small_block_kernel_general_shapes(ctx, dev, monkeypatch, prec):
             d_A, d_B, d_C = dev.put(A), dev.put(B), dev.put(C0)
             plan.execute(d_A.data_ptr(), d_B.data_ptr(), d_C.data_ptr(), 0.7, 0.3)
             got = dev.get(d_C, ref.shape)
+            info = plan.info
         finally:
             plan.close()
-        assert relerr(got, ref) < TOL64, case
+        assert info["tile"] == 10 and info["form"] == prec
+        # reduced forms: rounding error bounded by the form's unit times the
+        # sum of |terms| (a rank-0 result cancels far below its terms)
+        scale = contract(np.abs(A), labA, np.abs(B), labB, labC, alpha=0.7, beta=0.3, C=np.abs(C0))
+        err = np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), np.max(scale) if prec != "f64" else 0.0)
+        assert err < TOLS[prec], case
 
 
 SPLIT_CASES = [
@@ -245,8 +252,9 @@ SPLIT_CASES = [
 ]
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
 @pytest.mark.parametrize("case", range(len(SPLIT_CASES)))
-def test_split_small_plans(ctx, dev, case):
+def test_split_small_plans(ctx, dev, case, prec):
     """Small output blocks with a long contracted extent: the plan splits
     the k chunks over several workgroups and a reduction pass writes C
     (alpha, beta honoured); the result matches the oracle to 1e-12 and is
@@ -257,7 +265,7 @@ def test_split_small_plans(ctx, dev, case):
     B = rng.standard_normal(dB)
     C0 = rng.standard_normal(contract(A, labA, B, labB, labC).shape)
     ref = contract(A, labA, B, labB, labC, alpha=1.3, beta=-0.4, C=C0)
-    plan = ctx.plan_dense(A.shape, labA, B.shape, labB, ref.shape, labC, "f64")
+    plan = ctx.plan_dense(A.shape, labA, B.shape, labB, ref.shape, labC, prec)
     try:
         d_A, d_B = dev.put(A), dev.put(B)
         outs = []
@@ -267,7 +275,7 @@ def test_split_small_plans(ctx, dev, case):
             outs.append(dev.get(d_C, ref.shape))
     finally:
         plan.close()
-    assert relerr(outs[0], ref) < TOL64
+    assert relerr(outs[0], ref) < TOLS[prec]
     assert np.array_equal(outs[0], outs[1])
 
 

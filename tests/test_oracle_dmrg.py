@@ -325,3 +325,27 @@ def test_qn_dmrg_svd_route_vs_ed():
     E, _ = dmrg(Hm, psi0, sw, record=trace.append)
     assert {t["route"] for t in trace} == {"svd"}
     assert abs(E - GOLD["energies"]["10"]) < 1e-10
+
+
+@pytest.mark.parametrize("flux", [1, -2])
+def test_oracle_svd_nonzero_flux(flux):
+    """A QN tensor whose divergence is not zero (the orthogonality centre of
+    a particle-number MPS): U and V come out flux-free, D carries the
+    divergence on its diagonal blocks (the reference's QDiagReal, svd.cc:
+    321-410), U*D*V reconstructs the tensor and the singular values equal
+    numpy's on the dense expansion."""
+    from oracle.itensor import remove_qns
+
+    rng = np.random.default_rng(40 + flux)
+    L = Index([(QN(2), 3), (QN(1), 4), (QN(0), 5), (QN(-1), 4), (QN(-2), 2)], "L")
+    s = Index([(QN(1), 1), (QN(-1), 1)], "s")
+    R = Index([(QN(2), 2), (QN(1), 3), (QN(0), 6), (QN(-1), 3), (QN(-2), 3)], "R")
+    T = qdense([L, s, R.dag()], QN(flux), rng=rng)
+    U, D, V, spec = OD.svd(T, [L, s], {})
+    assert U.div() == QN() and V.div() == QN()
+    assert D.div() == QN(flux)
+    assert all(b[0] == b[1] for b, _ in D.offsets)
+    assert norm(U * D * V - T) < 1e-12 * norm(T)
+    Td = remove_qns(T).d.reshape(L.dim * s.dim, R.dim, order="F")
+    sv = np.linalg.svd(Td, compute_uv=False)
+    assert np.allclose(np.sort(spec["eigs"])[::-1], (sv ** 2)[: len(spec["eigs"])], atol=1e-12)
