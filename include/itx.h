@@ -127,6 +127,25 @@ int itx_contract_qn(itx_ctx *ctx,
                     const int64_t *blkdimsA, const int64_t *blkdimsB,
                     double alpha, int prec);
 
+/* As itx_contract_qn, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid it).  The call's task list, offset tables
+ * and split-K partial tiles go into a static device array of the library,
+ * 32 MiB per device, which each device holds until the process ends; a
+ * contraction whose plan needs more returns ITX_ERR_UNSUPPORTED (the
+ * message names both sizes).  The tile comes from the tuning table or the
+ * cost model, never from the autotuner.  Every call on a device shares
+ * that device's array: calls on one stream are ordered, calls that overlap
+ * on one device (other contexts or streams) race.  Nothing in the array
+ * needs to start at zero. */
+int itx_contract_qn_noalloc(itx_ctx *ctx,
+                            const double *A, int rA, const int32_t *labA,
+                            const double *B, int rB, const int32_t *labB,
+                            double *C, int rC, const int32_t *labC,
+                            int ntasks, const int64_t *offs,
+                            const int64_t *blkdimsA, const int64_t *blkdimsB,
+                            double alpha, int prec);
+
 /* ---- reusable plans (no host work or uploads at execution) ------------
  * A plan holds the device task list of one contraction structure; it is
  * executed any number of times on new data (e.g. the three H*psi products

@@ -1237,7 +1237,8 @@ std::string tune_key(int prec, bool akc, bool bkc, int maxM, int maxN, double ma
 
 int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t *labB, int rC,
                const int32_t *labC, int ntasks, const int64_t *offs, const int64_t *blkA,
-               const int64_t *blkB, const int64_t *blkC, int prec, itx_plan **out) {
+               const int64_t *blkB, const int64_t *blkC, int prec, itx_plan **out, void *into,
+               size_t into_cap) {
   *out = nullptr;
   if (prec != ITX_PREC_F64 && prec != ITX_PREC_F32 && prec != ITX_PREC_BF16X3)
     return set_err(ctx, ITX_ERR_ARG, "contraction: unknown precision form");
@@ -1600,15 +1601,27 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
       std::memcpy(host.data() + o_units, units.data(), bu_bytes);
       if (br_bytes) std::memcpy(host.data() + o_red, red.data(), br_bytes);
       if (!tab.empty()) std::memcpy(host.data() + o_tab, tab.data(), tab.size() * sizeof(int64_t));
-      int rc3 = pool_alloc(ctx, total, &p->dev);
-      if (rc3) {
-        delete p;
-        return rc3;
+      int rc3 = ITX_OK;
+      if (into) {
+        if (total > into_cap) {
+          delete p;
+          return set_err(ctx, ITX_ERR_UNSUPPORTED,
+                         "contraction: the plan needs " + std::to_string(total) + " bytes of device memory, more than the " +
+                             std::to_string(into_cap) + "-byte scratch of the allocation-free entry point");
+        }
+        p->dev = into;
+        p->own_dev = false;
+      } else {
+        rc3 = pool_alloc(ctx, total, &p->dev);
+        if (rc3) {
+          delete p;
+          return rc3;
+        }
       }
       p->ctx = ctx;
       rc3 = stage_upload(ctx, p->dev, host.data(), o_ws);
       if (rc3) {
-        pool_free(ctx, p->dev);
+        if (p->own_dev) pool_free(ctx, p->dev);
         delete p;
         return rc3;
       }
@@ -1651,7 +1664,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow)) best = 0;
   } else if (ctx->tune.count(key)) {
     best = ctx->tune[key];
-  } else if (ctx->autotune && !st.empty()) {
+  } else if (ctx->autotune && !into && !st.empty()) {
     void *bufA = nullptr, *bufB = nullptr, *bufC = nullptr;
     int rc2 = pool_alloc(ctx, sizeof(double) * size_t(extA), &bufA);
     if (!rc2) rc2 = pool_alloc(ctx, sizeof(double) * size_t(extB), &bufB);
@@ -1841,7 +1854,7 @@ int itx_plan_destroy(itx_ctx *ctx, itx_plan *plan) {
   // the device block goes back to the context's stream-ordered pool (no
   // wait); without a context it stays with the pool until itx_destroy
   if (ctx) {
-    if (plan->dev) itx::pool_free(ctx, plan->dev);
+    if (plan->dev && plan->own_dev) itx::pool_free(ctx, plan->dev);
     ctx->plans.erase(plan);
   }
   delete plan;
@@ -1979,6 +1992,23 @@ int itx_contract_qn(itx_ctx *ctx, const double *A, int rA, const int32_t *labA, 
   });
   if (rc) return rc;
   return itx::launch_plan(ctx, p, A, B, C, alpha, 0.0);
+}
+
+int itx_contract_qn_noalloc(itx_ctx *ctx, const double *A, int rA, const int32_t *labA, const double *B, int rB,
+                            const int32_t *labB, double *C, int rC, const int32_t *labC, int ntasks,
+                            const int64_t *offs, const int64_t *blkdimsA, const int64_t *blkdimsB, double alpha,
+                            int prec) {
+  if (!ctx) return ITX_ERR_ARG;
+  if (!ctx->scratch) return itx::set_err(ctx, ITX_ERR_UNSUPPORTED, "itx_contract_qn_noalloc: no scratch address");
+  // the plan lives in host memory for this call only; its device block is
+  // the scratch array, which the stream reuses in order for the next call
+  itx_plan *p = nullptr;
+  int rc = itx::build_plan(ctx, rA, labA, rB, labB, rC, labC, ntasks, offs, blkdimsA, blkdimsB, nullptr, prec, &p,
+                           ctx->scratch, ctx->scratch_bytes);
+  if (rc) return rc;
+  rc = itx::launch_plan(ctx, p, A, B, C, alpha, 0.0);
+  itx_plan_destroy(ctx, p);
+  return rc;
 }
 
 }  // extern "C"

@@ -76,6 +76,7 @@ struct itx_plan {
   const itx::WorkUnit *d_red = nullptr;
   const void *fn_red = nullptr;
   double *ws = nullptr;
+  bool own_dev = true;  // dev came from the pool (false: the caller's memory)
 };
 
 // A fixed sequence of plan executions replayed as one HIP graph (one or
@@ -105,6 +106,10 @@ struct itx_ctx {
   std::unordered_map<void *, size_t> host_live;
   // reduction scratch (partials + results)
   double *red = nullptr;
+  // device address of this device's copy of the library's static scratch
+  // array (itx_contract_qn_noalloc), from hipGetSymbolAddress at creation
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
   // pinned host memory: reduction results written by the device
   // (hres, device view hres_dev; coherent, with a completion flag after
   // the HRES_SLOTS result slots) and the staging arena of uploads
@@ -140,11 +145,16 @@ int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
 constexpr int HRES_SLOTS = 64;  // pinned result slots (multi-dot up to 16)
 
 // Build a contraction plan (dense = one task).  Shared by the dense and
-// QN entry points.  blk*: per-task block dims (ntasks x r).
+// QN entry points.  blk*: per-task block dims (ntasks x r).  With `into`
+// (device memory of into_cap bytes) the plan's device block -- task and
+// unit lists, offset tables, split-K partial tiles -- is laid out there
+// instead of in the caching pool, and the tile comes from the tuning table
+// or the cost model (the autotuner, which allocates, is skipped); a plan
+// that does not fit fails with ITX_ERR_UNSUPPORTED.
 int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t *labB,
                int rC, const int32_t *labC, int ntasks, const int64_t *offs,
                const int64_t *blkA, const int64_t *blkB, const int64_t *blkC_or_null,
-               int prec, itx_plan **out);
+               int prec, itx_plan **out, void *into = nullptr, size_t into_cap = 0);
 int launch_plan(itx_ctx *ctx, const itx_plan *p, const double *A, const double *B, double *C,
                 double alpha, double beta);
 // Arguments of launch `which` of a plan (0: the contraction, 1: the

@@ -89,6 +89,14 @@ int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
 namespace {
 constexpr size_t kStageBytes = size_t(8) << 20;
 
+// Device memory of itx_contract_qn_noalloc: one copy per device for the
+// life of the process, shared by every call on that device.  In host code
+// the name stands for a host-side shadow, not a device address: the address
+// comes from hipGetSymbolAddress (itx_create, with the device current) and
+// only that address is ever handed to kernels or copies.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 __global__ void k_fill(double *x, size_t n, double v) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t stride = size_t(gridDim.x) * blockDim.x;
@@ -154,6 +162,19 @@ int itx_create(int device, itx_ctx **out) {
   ctx->hres = static_cast<double *>(hp);
   ctx->hres_dev = static_cast<double *>(hd);
   ctx->stage_cap = kStageBytes;
+  // this device's copy of the static scratch array (the device is current)
+  void *sp = nullptr;
+  e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !sp) {
+    g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    hipHostFree(ctx->stage);
+    hipHostFree(hp);
+    hipFree(ctx->red);
+    delete ctx;
+    return ITX_ERR_HIP;
+  }
+  ctx->scratch = sp;
+  ctx->scratch_bytes = kScratchBytes;
   // ITX_AUTOTUNE=1 turns the tile autotuner on; ITX_TUNE_FILE names a
   // tuning table to load (see itx_tune_load)
   if (const char *a = std::getenv("ITX_AUTOTUNE")) ctx->autotune = std::atoi(a) ? 1 : 0;

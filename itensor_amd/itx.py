@@ -54,6 +54,8 @@ _PROTOS = {
                                      _vp, C.c_int, _i64p, _i32p, C.c_double, C.c_double, C.c_int]),
     "itx_contract_qn": (C.c_int, [_vp, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p,
                                   C.c_int, _i64p, _i64p, _i64p, C.c_double, C.c_int]),
+    "itx_contract_qn_noalloc": (C.c_int, [_vp, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p,
+                                          C.c_int, _i64p, _i64p, _i64p, C.c_double, C.c_int]),
     "itx_plan_dense": (C.c_int, [_vp, C.c_int, _i64p, _i32p, C.c_int, _i64p, _i32p, C.c_int, _i64p,
                                  _i32p, C.c_int, C.POINTER(_vp)]),
     "itx_plan_qn": (C.c_int, [_vp, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i64p,
@@ -180,17 +182,21 @@ class Context:
         self.check(rc, "itx_contract_dense")
 
     def contract_qn(self, A: int, labA, B: int, labB, Cp: int, labC, offs, blkA, blkB,
-                    alpha=1.0, prec="f64"):
+                    alpha=1.0, prec="f64", noalloc=False):
+        """itx_contract_qn, or with noalloc itx_contract_qn_noalloc (no
+        device allocation: the plan goes into the library's static scratch
+        array)."""
         offs = np.ascontiguousarray(offs, dtype=np.int64).ravel()
         blkA = np.ascontiguousarray(blkA, dtype=np.int64).ravel()
         blkB = np.ascontiguousarray(blkB, dtype=np.int64).ravel()
         nt = len(offs) // 3
-        rc = self.L.itx_contract_qn(
+        name = "itx_contract_qn_noalloc" if noalloc else "itx_contract_qn"
+        rc = getattr(self.L, name)(
             self.h, _vp(A), len(labA), _arr(C.c_int32, labA), _vp(B), len(labB), _arr(C.c_int32, labB),
             _vp(Cp), len(labC), _arr(C.c_int32, labC), nt,
             offs.ctypes.data_as(_i64p), blkA.ctypes.data_as(_i64p), blkB.ctypes.data_as(_i64p),
             float(alpha), PREC[prec])
-        self.check(rc, "itx_contract_qn")
+        self.check(rc, name)
 
     def plan_dense(self, dimsA, labA, dimsB, labB, dimsC, labC, prec="f64") -> "Plan":
         h = _vp()
