@@ -81,7 +81,9 @@ def build_libitx(force: bool = True) -> str:
         for f in [ex.submit(_run, [HIPCC, *cflags, "-c", s, "-o", o]) for s, o in todo]:
             f.result()
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    # rocSOLVER (with rocBLAS) diagonalises the density-matrix blocks above
+    # 256 rows
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-lrocsolver", "-lrocblas"])
     os.replace(tmp, out)
     return out
 

@@ -30,6 +30,8 @@
 #include <vector>
 
 #include <hip/hip_cooperative_groups.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "itx_internal.h"
 
@@ -1090,9 +1092,12 @@ class HostPool {
 
 }  // namespace
 
-int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, double *Abase,
-                 const int64_t *woffs, double *Wbase, double *whost) {
-  if (nmat <= 0) return ITX_OK;
+// Our kernels for every matrix of the call (n <= EMAX).  `between` runs once
+// the tridiagonal forms are on the host, before the QL recurrences: work it
+// enqueues runs on the device while the host iterates.
+int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, double *Abase, const int64_t *woffs,
+             double *Wbase, double *whost, const std::function<int()> &between) {
+  if (nmat <= 0) return between();
   // The kernels reduce every matrix to tridiagonal form and build Q; the
   // host runs the QL recurrences; k_syev_apply applies the recorded
   // rotations and writes the sorted eigenpairs.  Reduction size classes:
@@ -1127,7 +1132,7 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   std::vector<EigTask> all;
   for (auto *v : {&t64, &t128, &t256, &t512, &tg}) all.insert(all.end(), v->begin(), v->end());
   const int ntask = int(all.size());
-  if (ntask == 0) return ITX_OK;
+  if (ntask == 0) return between();
   for (auto &t : all) t.z_off += desize;
   const int64_t zsize = desize + qsize;
   const size_t tbytes = sizeof(EigTask) * size_t(ntask);
@@ -1210,6 +1215,10 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   if (e != hipSuccess) {
     pool_free(ctx, ws);
     return hip_err(ctx, e, "itx_syev");
+  }
+  if (int rb = between()) {
+    pool_free(ctx, ws);
+    return rb;
   }
   // The QL recurrences, one matrix per task on the host threads, largest
   // first; every task writes its own rotations, sweep records, eigenvalues
@@ -1314,6 +1323,195 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   pool_free(ctx, ub);
   pool_free(ctx, ws);
   return rc;
+}
+
+// ---- blocks above kOwnMax rows: rocSOLVER's divide-and-conquer dsyevd ----
+// ROCm's LAPACK in place of the reference's dsyev on -M: each block is
+// negated into a slot of its size class (N x N, column-major), whose
+// padding is c I with c = 1 + 2 |A|_F, above every eigenvalue of -A; the
+// tridiagonal form then splits exactly at the block edge, so the first n
+// ascending eigenpairs of the slot are those of -A: A's eigenvalues in
+// descending order (equal ones in LAPACK's order, as in the reference),
+// their eigenvectors zero outside the block.  One strided-batched call per
+// size class (blocks within a factor 1.26 of the class's largest, at most
+// twice the flops of unpadded blocks).
+constexpr int kOwnMax = 256;
+constexpr int EMAX_LARGE = 8192;
+
+struct PadTask {
+  int64_t a_off, p_off, w_off, d_off;  // A and w of the block; its slot and eigenvalues in the workspace
+  int32_t n, N;
+};
+
+__global__ __launch_bounds__(256) void k_pad_neg(const PadTask *__restrict__ tasks, const double *__restrict__ Abase,
+                                                 double *__restrict__ Pbase) {
+  __shared__ double red[4];
+  const PadTask t = tasks[blockIdx.x];
+  const double *__restrict__ A = Abase + t.a_off;
+  const int64_t nn = int64_t(t.n) * t.n, NN = int64_t(t.N) * t.N;
+  double s = 0;
+  for (int64_t x = threadIdx.x; x < nn; x += 256) s = fma(A[x], A[x], s);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double c = 1.0 + 2.0 * sqrt(red[0] + red[1] + red[2] + red[3]);
+  double *__restrict__ P = Pbase + t.p_off;
+  for (int64_t x = threadIdx.x; x < NN; x += 256) {
+    const int64_t col = x / t.N, row = x - col * t.N;
+    P[x] = (row < t.n && col < t.n) ? -A[row + col * t.n] : (row == col ? c : 0.0);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_unpad(const PadTask *__restrict__ tasks, const double *__restrict__ Pbase,
+                                               const double *__restrict__ Dbase, double *__restrict__ Abase,
+                                               double *__restrict__ Wbase) {
+  const PadTask t = tasks[blockIdx.x];
+  const double *__restrict__ P = Pbase + t.p_off;
+  double *__restrict__ A = Abase + t.a_off;
+  const int64_t nn = int64_t(t.n) * t.n;
+  for (int64_t x = threadIdx.x; x < nn; x += 256) {
+    const int64_t col = x / t.n, row = x - col * t.n;
+    A[x] = P[row + col * t.N];
+  }
+  for (int k = threadIdx.x; k < t.n; k += 256) Wbase[t.w_off + k] = -Dbase[t.d_off + k];
+}
+
+struct LargeCall {
+  void *ws = nullptr;
+  int *info = nullptr;
+  int count = 0;
+  std::vector<PadTask> tasks;
+};
+
+rocblas_handle solver_handle(itx_ctx *ctx) {
+  if (!ctx->rocblas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+    ctx->rocblas = h;
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(ctx->rocblas);
+  rocblas_set_stream(h, ctx->stream);
+  return h;
+}
+
+int syev_large_launch(itx_ctx *ctx, std::vector<PadTask> tasks, double *Abase, double *Wbase, LargeCall &lc) {
+  if (tasks.empty()) return ITX_OK;
+  rocblas_handle h = solver_handle(ctx);
+  if (!h) return set_err(ctx, ITX_ERR_HIP, "itx_syev: rocblas_create_handle failed");
+  // size classes, largest first
+  std::stable_sort(tasks.begin(), tasks.end(), [](const PadTask &a, const PadTask &b) { return a.n > b.n; });
+  struct Cls {
+    int N, first, count;
+  };
+  std::vector<Cls> cls;
+  int64_t pbytes = 0, dsz = 0;
+  for (int i = 0; i < int(tasks.size());) {
+    const int N = (tasks[size_t(i)].n + 31) & ~31;
+    int j = i;
+    while (j < int(tasks.size()) && double(tasks[size_t(j)].n) * 1.26 >= double(N)) {
+      tasks[size_t(j)].N = N;
+      tasks[size_t(j)].p_off = pbytes + int64_t(j - i) * N * N;
+      tasks[size_t(j)].d_off = dsz + int64_t(j - i) * N;
+      ++j;
+    }
+    cls.push_back({N, i, j - i});
+    pbytes += int64_t(j - i) * N * N;
+    dsz += int64_t(j - i) * N;
+    i = j;
+  }
+  // workspace: tasks | slots | eigenvalues | off-diagonals | info
+  const size_t tb = sizeof(PadTask) * tasks.size();
+  const size_t o_p = (tb + 255) & ~size_t(255);
+  const size_t o_d = o_p + sizeof(double) * size_t(pbytes);
+  const size_t o_e = o_d + sizeof(double) * size_t(dsz);
+  const size_t o_i = o_e + sizeof(double) * size_t(dsz);
+  int rc = pool_alloc(ctx, o_i + sizeof(int) * tasks.size(), &lc.ws);
+  if (rc) return rc;
+  char *w = static_cast<char *>(lc.ws);
+  rc = stage_upload(ctx, w, tasks.data(), tb);
+  if (rc) return rc;
+  const PadTask *dt = reinterpret_cast<const PadTask *>(w);
+  double *P = reinterpret_cast<double *>(w + o_p), *D = reinterpret_cast<double *>(w + o_d),
+         *E = reinterpret_cast<double *>(w + o_e);
+  lc.info = reinterpret_cast<int *>(w + o_i);
+  lc.count = int(tasks.size());
+  hipLaunchKernelGGL(k_pad_neg, dim3(unsigned(tasks.size())), dim3(256), 0, ctx->stream, dt, Abase, P);
+  for (const Cls &c : cls) {
+    const PadTask &t0 = tasks[size_t(c.first)];
+    const rocblas_status st = rocsolver_dsyevd_strided_batched(
+        h, rocblas_evect_original, rocblas_fill_lower, c.N, P + t0.p_off, c.N, rocblas_stride(c.N) * c.N,
+        D + t0.d_off, c.N, E + t0.d_off, c.N, lc.info + c.first, c.count);
+    if (st != rocblas_status_success) return set_err(ctx, ITX_ERR_HIP, "itx_syev: rocsolver_dsyevd failed");
+  }
+  hipLaunchKernelGGL(k_unpad, dim3(unsigned(tasks.size())), dim3(256), 0, ctx->stream, dt, P, D, Abase, Wbase);
+  lc.tasks = std::move(tasks);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? ITX_OK : hip_err(ctx, e, "itx_syev (large blocks)");
+}
+
+// waits for the large blocks: convergence flags, and their eigenvalues to
+// the host when asked for
+int syev_large_finish(itx_ctx *ctx, LargeCall &lc, const double *Wbase, double *whost) {
+  if (!lc.count) return ITX_OK;
+  std::vector<int> info(size_t(lc.count));
+  hipError_t e = hipMemcpyAsync(info.data(), lc.info, sizeof(int) * info.size(), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && whost)
+    for (const PadTask &t : lc.tasks) {
+      e = hipMemcpyAsync(whost + t.w_off, Wbase + t.w_off, sizeof(double) * size_t(t.n), hipMemcpyDeviceToHost,
+                         ctx->stream);
+      if (e != hipSuccess) break;
+    }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  pool_free(ctx, lc.ws);
+  lc.ws = nullptr;
+  if (e != hipSuccess) return hip_err(ctx, e, "itx_syev (large blocks)");
+  for (int x : info)
+    if (x != 0) return set_err(ctx, ITX_ERR_NOCONV, "itx_syev: dsyevd did not converge");
+  return ITX_OK;
+}
+
+// Every matrix of a call: blocks up to kOwnMax rows (all of them with
+// ITX_EIG_OWN=1, up to EMAX) through our kernels, larger ones through
+// rocSOLVER, enqueued while the host runs the QL recurrences of the others.
+int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, double *Abase,
+                 const int64_t *woffs, double *Wbase, double *whost) {
+  if (nmat <= 0) return ITX_OK;
+  static const bool own_only = [] {
+    const char *v = std::getenv("ITX_EIG_OWN");
+    return v && std::atoi(v) != 0;
+  }();
+  std::vector<int32_t> on;
+  std::vector<int64_t> oa, ow;
+  std::vector<PadTask> large;
+  for (int b = 0; b < nmat; ++b) {
+    const int n = ns[b];
+    if (n < 0 || n > (own_only ? EMAX : EMAX_LARGE))
+      return set_err(ctx, ITX_ERR_UNSUPPORTED,
+                     own_only ? "itx_syev: n above 2048 (ITX_EIG_OWN)" : "itx_syev: n above 8192");
+    if (n == 0) continue;
+    if (own_only || n <= kOwnMax) {
+      on.push_back(n);
+      oa.push_back(aoffs[b]);
+      ow.push_back(woffs[b]);
+    } else {
+      PadTask t{};
+      t.a_off = aoffs[b];
+      t.w_off = woffs[b];
+      t.n = n;
+      large.push_back(t);
+    }
+  }
+  LargeCall lc;
+  int rc = syev_own(ctx, int(on.size()), on.data(), oa.data(), Abase, ow.data(), Wbase, whost,
+                    [&] { return syev_large_launch(ctx, std::move(large), Abase, Wbase, lc); });
+  const int rc2 = syev_large_finish(ctx, lc, Wbase, whost);
+  return rc ? rc : rc2;
+}
+
+void eig_release(itx_ctx *ctx) {
+  if (ctx->rocblas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rocblas));
+  ctx->rocblas = nullptr;
 }
 
 }  // namespace itx

@@ -110,6 +110,8 @@ struct itx_ctx {
   // array (itx_contract_qn_noalloc), from hipGetSymbolAddress at creation
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // rocblas handle of the large-block eigensolver (created on first use)
+  void *rocblas = nullptr;
   // pinned host memory: reduction results written by the device
   // (hres, device view hres_dev; coherent, with a completion flag after
   // the HRES_SLOTS result slots) and the staging arena of uploads
@@ -132,6 +134,8 @@ struct itx_ctx {
 namespace itx {
 
 int set_err(itx_ctx *ctx, int code, const std::string &msg);
+// releases the eigensolver's library handle (itx_destroy)
+void eig_release(itx_ctx *ctx);
 int hip_err(itx_ctx *ctx, hipError_t e, const char *where);
 
 // Device allocation through the context's caching pool.

@@ -191,6 +191,7 @@ int itx_destroy(itx_ctx *ctx) {
   while (!ctx->chains.empty()) itx_chain_destroy(ctx, *ctx->chains.begin());
   ctx->plan_cache.clear();
   while (!ctx->plans.empty()) itx_plan_destroy(ctx, *ctx->plans.begin());
+  itx::eig_release(ctx);
   for (auto &kv : ctx->free_blocks) hipFree(kv.second);
   for (auto &kv : ctx->live) hipFree(kv.first);
   hipFree(ctx->red);

@@ -55,15 +55,15 @@ def test_permute_add(ctx, dev, dims, perm):
     assert relerr(dev.get(dY, dims), np.transpose(X, perm)) == 0.0
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 10, 64, 65, 100, 128, 129, 256, 257, 400, 512, 513])
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 64, 65, 100, 128, 129, 256, 257, 400, 512, 513, 1024, 2048])
 def test_syev(ctx, dev, n):
-    """Single matrices of every size class (reduction LDS-resident up to 64
-    and up to 128, in global memory above; rotations applied to Q rows
-    staged in LDS up to 256, in global memory above): eigenvalues vs numpy,
-    orthonormal vectors, reconstruction, descending order."""
+    """Single matrices of every size class (our kernels up to 256 rows:
+    reduction LDS-resident up to 64 and up to 128, in global memory above;
+    rocSOLVER's dsyevd on the negated, padded block above): eigenvalues vs
+    numpy, orthonormal vectors, reconstruction, descending order."""
     rng = np.random.default_rng(1000 + n)
     X = rng.standard_normal((n, n))
-    if n in (64, 128, 256, 512):
+    if n in (64, 128, 256, 512, 2048):
         # density-matrix-like: PSD with exact zero eigenvalues and degeneracies
         X = X[:, :20] @ X[:, :20].T
         X[:, :2] = 0
@@ -200,3 +200,41 @@ def test_gesvd_batched(ctx, dev):
         V = Vh[vo: vo + b * k].reshape((b, k), order="F")
         S = Sh[so: so + k]
         assert np.linalg.norm(U @ np.diag(S) @ V.T - M) < 1e-12 * max(1.0, np.linalg.norm(M))
+
+
+def test_syev_batched_large_size_classes(ctx, dev):
+    """Several blocks above 256 rows in one call: they go to the large-block
+    path in size classes (padded slots of 320, 512, 704 and 1024 rows here),
+    beside small blocks on our kernels; density-matrix-like blocks with
+    exact zero eigenvalues included.  Every block against numpy."""
+    rng = np.random.default_rng(79)
+    ns = [300, 20, 301, 257, 700, 1024, 0, 500, 450, 130]
+    mats = []
+    for k, n in enumerate(ns):
+        if k % 2:
+            Y = rng.standard_normal((n, max(1, n // 3)))
+            X = Y @ Y.T
+        else:
+            X = rng.standard_normal((n, n))
+            X = X + X.T
+        mats.append(X)
+    a_offs = np.cumsum([0] + [n * n for n in ns])[:-1]
+    w_offs = np.cumsum([0] + ns)[:-1]
+    flat = np.concatenate([m.ravel(order="F") for m in mats] + [np.zeros(1)])
+    dA = dev.put(flat)
+    dw = dev.empty(sum(ns) + 1)
+    wh = ctx.syev_batched_host(ns, a_offs, dA.data_ptr(), w_offs, dw.data_ptr())
+    A = dev.get(dA, (flat.size,))
+    w = dev.get(dw, (sum(ns) + 1,))
+    assert np.array_equal(wh[: sum(ns)], w[: sum(ns)])
+    for X, n, ao, wo in zip(mats, ns, a_offs, w_offs):
+        if n == 0:
+            continue
+        U = A[ao: ao + n * n].reshape((n, n), order="F")
+        wn = w[wo: wo + n]
+        wref = np.linalg.eigvalsh(X)[::-1]
+        scale = max(1.0, np.max(np.abs(wref)))
+        assert np.max(np.abs(wn - wref)) / scale < 1e-12, n
+        assert np.all(np.diff(wn) <= 0), n
+        assert np.max(np.abs(U.T @ U - np.eye(n))) < 1e-12, n
+        assert np.max(np.abs(U @ np.diag(wn) @ U.T - X)) / scale < 1e-12, n
