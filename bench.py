@@ -43,7 +43,7 @@ METRIC = "DMRG sec/sweep + Hψ TFLOP/s, mixed vs FP64 on 1 MI355X at |ΔE|≤1e-
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
 SEED = 20261202 + 2
 
-from itensor_amd.workloads import (DMRG_MAXDIM, FUSED, HUBBARD_MAXDIM, SECTORS_1024, STAGES,  # noqa: E402
+from itensor_amd.workloads import (DMRG_BIG_MAXDIM, DMRG_MAXDIM, FUSED, HUBBARD_MAXDIM, SECTORS_1024, STAGES,  # noqa: E402
                                    fused_dims, heisenberg_qn_chain, heisenberg_w, hpsi_flops, hubbard_cylinder,
                                    qn_operands, stage_dims)
 
@@ -99,8 +99,10 @@ def qn_hpsi(steps: int, warmup: int):
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) / steps * 1e3
         st = out.store()
-        r = {"ms_per_hpsi": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 3), "flops": flops,
-             "first_call_ms": round(t_plan * 1e3, 2)}
+        tf = flops / (ms * 1e-3) / 1e12
+        form = ("f64", "f32", "bf16x3")[prec]
+        r = {"ms_per_hpsi": round(ms, 4), "tflops": round(tf, 3), "flops": flops,
+             "frac_of_peak": round(tf / PEAK[form], 4), "first_call_ms": round(t_plan * 1e3, 2)}
         if ref is None:
             ref = (st, out.offsets)
             r["blocks_out"] = len(out.offsets)
@@ -295,8 +297,9 @@ def main():
                     help="Hubbard cylinder leg: NxxNy:maxdim1,maxdim2,... ('' skips it)")
     ap.add_argument("--hubbard-forms", default="f64,f32,bf16x3+f64last",
                     help="H*psi forms of the Hubbard leg ('+f64last': the last sweep in FP64)")
-    ap.add_argument("--dmrg-big", default="100:full",
-                    help="QN DMRG leg with MinDim = MaxDim (bond dimension follows the ramp to 2048); '' skips it")
+    ap.add_argument("--dmrg-big", default="100:" + ",".join(str(m) for m in DMRG_BIG_MAXDIM),
+                    help="QN DMRG leg with MinDim = MaxDim (the bond dimension follows the ramp; 'full' = the ramp "
+                         "to 2048); '' skips it")
     ap.add_argument("--dmrg-big-forms", default="f64,f32,bf16x3",
                     help="H*psi forms of the MinDim = MaxDim DMRG leg")
     ap.add_argument("--phase-split", action="store_true",
@@ -420,8 +423,16 @@ def main():
     }
     for prec, r in results.items():
         line["stages_ms"][prec] = [round(x, 4) for x in r["stage_ms"]]
+        sm = r["stage_ms"]
+        dk = int(np.argmax(sm))
+        ach = r["stage_flops"][dk] / (sm[dk] * 1e-3) / 1e12
         line["forms"][prec] = {"tflops": round(flops / (r["wall_s"] / args.steps) / 1e12, 3),
-                               "ms_per_hpsi": round(r["wall_s"] / args.steps * 1e3, 4)}
+                               "ms_per_hpsi": round(r["wall_s"] / args.steps * 1e3, 4),
+                               # the form's dominant stage against the peak of the arithmetic it runs on
+                               # (BF16x3: three BF16 products per FP64-equivalent one)
+                               "roofline": {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK[prec],
+                                            "unit": "TFLOP/s (FP64-equivalent)", "frac": round(ach / PEAK[prec], 4),
+                                            "stage": dk + 1, "avg_launch_ms": round(sm[dk], 4)}}
 
     cores, nproc, cpu_model = host_cpu()
     cpu_model = "%s (%d cores visible)" % (cpu_model, nproc)

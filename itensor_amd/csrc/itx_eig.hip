@@ -825,141 +825,156 @@ __global__ __launch_bounds__(ES) void k_syev_lds(const EigTask *__restrict__ tas
 }
 
 // ---- hybrid form: rotations recorded by the host, applied here ------------
-// Every QL sweep is a chain of plane rotations on consecutive index pairs
-// (i, i+1), i = top, top-1, ..., lo, each applied to every row z of Q as
-//   z[i+1] <- s z[i] + c t,   t <- c z[i] - s t,   t starting at z[top+1]
-// and z[lo] <- t at the end (t: the element the chain carries down).
-//
-// Wavefront apply: WK consecutive sweeps go over a row together.  At step p
-// sweep k works at i = p + k, so that sweep k+1 at i+1 takes the element
-// sweep k has just finished; a sweep outside its range is the rotation
-// (c, s) = (1, 0), a one-element delay (z[i+1] <- t, t <- z[i]), which makes
-// every step uniform.  Each lane holds WK carried elements of its row in
-// registers and reads and writes one element of the row per step instead of
-// one per rotation.  Steps are skewed (sweep k runs step p at diagonal
-// p0 - p + k), so the WK rotations of a diagonal are independent.  The
-// (c, s) of WD diagonals x WK sweeps are staged in LDS by the workgroup
-// (one per thread), double buffered.
-struct SweepRec {
-  int32_t top, lo, off, pad;  // rotations i = top .. lo at R[off + (top - i)]
+// Every QL sweep is a chain of rotations on consecutive index pairs
+// (i, i+1), i = top, top-1, ..., top-cnt+1.  The host sends one stream of
+// double pairs per matrix: a header (top, cnt) per sweep followed by that
+// sweep's (c, s) pairs in application order.
+struct RotMeta {
+  int64_t rot_off, ev_off;      // the stream; eigenvalues then ranks
+  int64_t a_off, w_off, z_off;  // A, w; Q (column-major, n x (n+1)) in the workspace
+  int32_t n, nrot;              // nrot: stream length (headers included)
 };
-struct WaveMeta {
-  int64_t rot_off, sw_off, ev_off;  // rotations, sweep records; eigenvalues then ranks
-  int64_t a_off, w_off, z_off;      // A, w; Q (column-major, n x (n+1)) in the workspace
-  int32_t n, nsw;
-};
-constexpr int WK = 16;       // sweeps per wavefront
-constexpr int WT = 256;      // rows per apply workgroup (one row per thread)
-constexpr int WD = WT / WK;  // diagonals per staged chunk: one (c, s) per thread
+constexpr int AT = 64;    // rows per apply block (one wave, one row per lane)
+constexpr int AK = 16;    // rotations per straight-line chunk (the longest)
+constexpr int RB = 1024;  // rotations staged in LDS per batch
 
-// Q <- Q G_1 G_2 ..., then this workgroup's rows of the sorted eigenvectors
-// (column rank[c] of A is column c of Q) and, from the first workgroup of a
-// matrix, the sorted eigenvalues.
-__global__ __launch_bounds__(WT) void k_syev_wave(const WaveMeta *__restrict__ meta,
-                                                  const SweepRec *__restrict__ sweeps,
-                                                  const double2 *__restrict__ rots, const double *__restrict__ evr,
-                                                  double *__restrict__ Abase, double *__restrict__ Wbase,
-                                                  double *__restrict__ Zbase) {
-  __shared__ double2 Cs[2][WD * WK];
-  __shared__ double Xs[2][WD][WT];
-  __shared__ int Gs[3][WK];
-  const WaveMeta mt = meta[blockIdx.x];
-  const int n = mt.n;
-  const int r0 = blockIdx.y * WT;
-  if (r0 >= n) return;  // whole workgroup
-  const int tid = threadIdx.x;
-  const int r = min(r0 + tid, n - 1);  // threads past the last row redo row n-1 and store nothing
-  const bool act = r0 + tid < n;
-  double *__restrict__ Z = Zbase + mt.z_off + r;  // element (r, p) at Z[p * n]
-  const double2 *__restrict__ R = rots + mt.rot_off;
-  const SweepRec *__restrict__ SW = sweeps + mt.sw_off;
-  const int sd = tid / WK, ss = tid % WK;  // this thread's staging entry (diagonal, sweep)
-  for (int g0 = 0; g0 < mt.nsw; g0 += WK) {
-    const int ns = min(WK, mt.nsw - g0);
-    __syncthreads();  // the previous group is done with Gs and Cs
-    if (tid < WK) {
-      // absent sweeps: lo > top, never active
-      const SweepRec q = tid < ns ? SW[g0 + tid] : SweepRec{-2, 0, 0, 0};
-      Gs[0][tid] = q.top;
-      Gs[1][tid] = q.lo;
-      Gs[2][tid] = q.off;
+// Applies the rotation stream to this lane's row of Q (element (row, i) at
+// zc[i * ld]).  Within a sweep position i is read once, before the rotation
+// on (i, i+1), and the rotation on (i+1, i+2) has left only i+1 changed, so
+// a chunk loads its positions together ahead of its rotations; the updated
+// element i travels to the next rotation in a register and each rotation
+// stores position i+1, which the chain does not touch again.  A sweep runs
+// as chunks of 16 rotations, its remainder as chunks of 8, 4, 2 and 1, so
+// no chunk reaches past the sweep and none needs a mask.
+template <int LDC>  // LDC > 0: Q rows in LDS with that leading dimension (32-bit offsets)
+__device__ __forceinline__ void apply_stream(double *zc, int ld, int n, const RotMeta &mt,
+                                             const double2 *__restrict__ R, double2 *Rs) {
+  auto at = [&](int p) -> double & {
+    if constexpr (LDC > 0)
+      return zc[p * LDC];
+    else
+      return zc[size_t(p) * ld];
+  };
+  int segrem = 0, i = 0;
+  double carry = 0;
+  for (int b0 = 0; b0 < mt.nrot; b0 += RB) {
+    const int qn = min(RB, mt.nrot - b0);
+    double2 ld_[RB / AT];  // all loads of the batch in flight at once
+#pragma unroll
+    for (int u = 0; u < RB / AT; ++u) {
+      const int x = u * AT + threadIdx.x;
+      ld_[u] = x < qn ? R[b0 + x] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    int hi = -1, lo = 1 << 30;
 #pragma unroll
-    for (int k = 0; k < WK; ++k)
-      if (k < ns) {
-        hi = max(hi, Gs[0][k]);
-        lo = min(lo, Gs[1][k]);
+    for (int u = 0; u < RB / AT; ++u) Rs[u * AT + threadIdx.x] = ld_[u];
+    __syncthreads();
+    int q = 0;
+    // C consecutive rotations of the current sweep, positions i .. i-C+1
+    // (all inside the sweep, so no clamp or mask): every load ahead of the
+    // stores, offsets from the chunk base compile-time constants
+    auto chunk = [&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      double *zb = &at(i - (C - 1));  // position i - t is zb[(C - 1 - t) * step]
+      double z[C];
+      double2 g[C];
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        if constexpr (LDC > 0)
+          z[t] = zb[(C - 1 - t) * LDC];
+        else
+          z[t] = zb[size_t(C - 1 - t) * ld];
+        g[t] = Rs[q + t];
       }
-    const int p0 = hi + 1;        // first step: every sweep still before its range
-    const int pend = lo - WK;     // last step of sweep 0
-    const int D = p0 - pend + WK;  // diagonals until sweep WK-1 has done step pend
-    const int s_top = Gs[0][ss], s_lo = Gs[1][ss], s_off = Gs[2][ss];
-    auto stage = [&](int buf, int d0) {
-      const int i = p0 - (d0 + sd) + 2 * ss;  // sweep ss at diagonal d0 + sd
-      Cs[buf][tid] = (i >= s_lo && i <= s_top) ? R[s_off + (s_top - i)] : make_double2(1.0, 0.0);
+#pragma unroll
+      for (int t = 0; t < C; ++t) {
+        const double cz = g[t].x * z[t];
+        const double up = fma(g[t].y, z[t], g[t].x * carry);
+        carry = fma(-g[t].y, carry, cz);
+        if constexpr (LDC > 0)
+          zb[(C - t) * LDC] = up;  // position i - t + 1
+        else
+          zb[size_t(C - t) * ld] = up;
+      }
+      q += C;
+      segrem -= C;
+      i -= C;
     };
-    double xin[WK], t[WK];
-#pragma unroll
-    for (int k = 0; k < WK; ++k) xin[k] = t[k] = 0.0;  // flushed out above hi + 1, never stored
-    // The row elements sweep 0 takes in go through this thread's LDS slots
-    // one chunk of WD diagonals ahead: a chunk's loads are issued before its
-    // rotations and land in LDS after them (diagonals past D only carry
-    // delays: loads below pend are skipped).
-    double xn[WD];
-    auto fetch = [&](int d0) {
-#pragma unroll
-      for (int u = 0; u < WD; ++u) {
-        const int p = p0 - (d0 + u);
-        xn[u] = (p >= pend && p >= 0) ? Z[size_t(p) * n] : 0.0;
+    while (q < qn) {
+      if (segrem == 0) {  // a sweep header
+        const double2 h = Rs[q++];
+        i = int(h.x);
+        segrem = int(h.y);
+        carry = at(i + 1);
+        continue;  // the header may end the batch
       }
-    };
-    auto park = [&](int buf) {
-#pragma unroll
-      for (int u = 0; u < WD; ++u) Xs[buf][u][tid] = xn[u];
-    };
-    stage(0, 0);
-    fetch(0);
-    park(0);
-    int buf = 0;
-    for (int d0 = 0; d0 < D; d0 += WD) {
-      __syncthreads();  // Cs[buf] is staged; Cs[buf ^ 1] is no longer read
-      const bool more = d0 + WD < D;
-      if (more) {
-        stage(buf ^ 1, d0 + WD);
-        fetch(d0 + WD);
-      }
-#pragma unroll 1
-      for (int dd = 0; dd < WD; ++dd) {
-        const int p = p0 - (d0 + dd);
-        xin[0] = Xs[buf][dd][tid];
-        const double2 *cf = &Cs[buf][dd * WK];
-#pragma unroll
-        for (int k = WK - 1; k >= 0; --k) {
-          const double2 g = cf[k];
-          const double x = xin[k];
-          const double up = fma(g.y, x, g.x * t[k]);
-          t[k] = fma(g.x, x, -g.y * t[k]);
-          if (k == WK - 1) {
-            const int pos = p + 2 * WK - 1;  // the last sweep's finished element
-            if (act && pos >= lo && pos <= hi + 1) Z[size_t(pos) * n] = up;
-          } else {
-            xin[k + 1] = up;
-          }
-        }
-      }
-      if (more) park(buf ^ 1);  // this thread's slots of the other buffer are free
-      buf ^= 1;
+      // the sweep's rotations in this batch, in chunks of 16, then 8, 4, 2, 1
+      const int avail = min(segrem, qn - q);
+      if (avail >= AK)
+        chunk(std::integral_constant<int, AK>{});
+      else if (avail >= 8)
+        chunk(std::integral_constant<int, 8>{});
+      else if (avail >= 4)
+        chunk(std::integral_constant<int, 4>{});
+      else if (avail >= 2)
+        chunk(std::integral_constant<int, 2>{});
+      else
+        chunk(std::integral_constant<int, 1>{});
+      if (segrem == 0) at(i + 1) = carry;
     }
   }
-  // the sorted output: column rank[c] of A is column c of Q
+}
+
+// Q <- Q G_1 G_2 ..., then this block's rows of the sorted eigenvectors
+// (column rank[c] of A is column c of Q) and, from the first block of a
+// matrix, the sorted eigenvalues.  LC > 0: the block's 64 rows of Q are
+// staged in LDS (n <= LC); LC == 0: they are updated in the workspace.
+template <int LC>
+__global__ __launch_bounds__(AT) void k_syev_apply(const RotMeta *__restrict__ meta,
+                                                   const double2 *__restrict__ rots, const double *__restrict__ evr, double *__restrict__ Abase,
+                                                   double *__restrict__ Wbase, double *__restrict__ Zbase) {
+  __shared__ double2 Rs[RB + AK];
+  __shared__ double Zs[LC > 0 ? (LC + 1) * AT : 1];
+  const RotMeta mt = meta[blockIdx.x];
+  const int n = mt.n;
+  const int r0 = blockIdx.y * AT;
+  if (r0 >= n) return;
+  const int lane = threadIdx.x;
+  const int r = min(r0 + lane, n - 1);  // lanes past the last row redo row n-1 (same values)
+  const bool act = r0 + lane < n;
+  const double *Zg = Zbase + mt.z_off + r;
+  const double2 *R = rots + mt.rot_off;
+  double *zc;
+  int ld;
+  if constexpr (LC > 0) {
+    zc = Zs + lane;
+    ld = AT;
+    for (int c0 = 0; c0 < n; c0 += 16) {  // sixteen loads in flight
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = c0 + u < n ? Zg[size_t(c0 + u) * n] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (c0 + u < n) zc[(c0 + u) * AT] = v[u];
+    }
+  } else {
+    zc = Zbase + mt.z_off + r;
+    ld = n;
+  }
+  apply_stream<(LC > 0 ? AT : 0)>(zc, ld, n, mt, R, Rs);
+  // ranks to LDS (the rotation buffer is free now), then the sorted output
   const double *ev = evr + mt.ev_off, *rk = ev + n;
-  if (blockIdx.y == 0)
-    for (int c = tid; c < n; c += WT) Wbase[mt.w_off + int(rk[c])] = ev[c];
+  int *rks = reinterpret_cast<int *>(Rs);
+  __syncthreads();
+  for (int c = lane; c < n; c += AT) {
+    const int k = int(rk[c]);
+    rks[c] = k;
+    if (blockIdx.y == 0) Wbase[mt.w_off + k] = ev[c];
+  }
+  __syncthreads();
   if (!act) return;
   double *__restrict__ A = Abase + mt.a_off + r;
-  for (int c = 0; c < n; ++c) A[size_t(int(rk[c])) * n] = Z[size_t(c) * n];
+  for (int c = 0; c < n; ++c) A[size_t(rks[c]) * n] = zc[size_t(c) * ld];
 }
 
 // sqrt(a^2 + b^2) without overflow or underflow of the squares
@@ -974,10 +989,10 @@ inline double pyth(double a, double b) {
 
 // Implicit QL with the tql2 deflation test on the host (the recurrence is
 // a serial chain of a few dependent flops per rotation, which one CPU core
-// runs an order of magnitude faster than one GPU lane).  Appends every
-// sweep's (c, s) pairs to rots and its record (top, lo, offset into rots)
-// to sweeps; d ends up holding the eigenvalues.
-bool host_ql(int n, double *d, double *e, std::vector<double2> &rots, std::vector<SweepRec> &sweeps) {
+// runs an order of magnitude faster than one GPU lane).  Appends a header
+// (top, cnt) and the (c, s) pairs of every sweep to the stream; d ends up
+// holding the eigenvalues.
+bool host_ql(int n, double *d, double *e, std::vector<double2> &rots) {
   double tst1 = 0;
   for (int l = 0; l < n; ++l) {
     int iter = 0;
@@ -993,7 +1008,8 @@ bool host_ql(int n, double *d, double *e, std::vector<double2> &rots, std::vecto
       g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
       double s = 1, c = 1, p = 0;
       bool early = false;
-      const size_t first = rots.size();
+      const size_t hdr = rots.size();
+      rots.push_back(make_double2(0.0, 0.0));
       for (int i = m - 1; i >= l; --i) {
         const double f = s * e[i], b = c * e[i];
         r = pyth(f, g);
@@ -1013,8 +1029,11 @@ bool host_ql(int n, double *d, double *e, std::vector<double2> &rots, std::vecto
         g = c * r - b;
         rots.push_back(make_double2(c, s));
       }
-      const int cnt = int(rots.size() - first);
-      if (cnt) sweeps.push_back(SweepRec{m - 1, m - cnt, int32_t(first), 0});
+      const size_t cnt = rots.size() - hdr - 1;
+      if (cnt)
+        rots[hdr] = make_double2(double(m - 1), double(cnt));
+      else
+        rots.pop_back();
       if (!early) {
         d[l] -= p;
         e[l] = g;
@@ -1224,19 +1243,16 @@ int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, do
   // first; every task writes its own rotations, sweep records, eigenvalues
   // and descending ranks, concatenated in matrix order afterwards.
   struct QlOut {
-    std::vector<double2> rots;
-    std::vector<SweepRec> sw;
-    std::vector<double> evr;  // eigenvalues (QL order), then ranks
+    std::vector<double2> rots;  // the rotation stream (a header per sweep)
+    std::vector<double> evr;    // eigenvalues (QL order), then ranks
     bool ok = true;
   };
   std::vector<QlOut> outs(static_cast<size_t>(ntask));
   std::vector<int> order(static_cast<size_t>(ntask));
   double work = 0;
-  int maxn = 0;
   for (int b = 0; b < ntask; ++b) {
     order[size_t(b)] = b;
     work += double(all[size_t(b)].n) * all[size_t(b)].n;
-    maxn = std::max(maxn, all[size_t(b)].n);
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return all[size_t(a)].n > all[size_t(b)].n; });
   auto ql_task = [&](int j) {
@@ -1245,8 +1261,7 @@ int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, do
     const int n = t.n;
     double *d = de.data() + t.de_off, *ee = d + n;
     QlOut &o = outs[size_t(b)];
-    o.rots.reserve(size_t(n) * size_t(n));
-    o.ok = host_ql(n, d, ee, o.rots, o.sw);
+    o.ok = host_ql(n, d, ee, o.rots);
     if (!o.ok) return;
     // stable descending rank: larger first, equal values in QL order
     std::vector<int> idx(static_cast<size_t>(n));
@@ -1264,8 +1279,10 @@ int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, do
     HostPool::get().run(ntask, ql_task);
   else
     for (int j = 0; j < ntask; ++j) ql_task(j);
-  std::vector<WaveMeta> meta(static_cast<size_t>(ntask));
-  size_t nrot = 0, nsw = 0, nev = 0;
+  // apply classes: Q rows staged in LDS for n <= 64, 128, 256; else global
+  std::vector<RotMeta> cls[4];
+  int maxn[4] = {0, 0, 0, 0};
+  size_t nrot = 0, nev = 0;
   for (int b = 0; b < ntask; ++b) {
     const QlOut &o = outs[size_t(b)];
     if (!o.ok) {
@@ -1273,32 +1290,31 @@ int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, do
       return set_err(ctx, ITX_ERR_NOCONV, "itx_syev: QL iteration did not converge");
     }
     const EigTask &t = all[size_t(b)];
-    WaveMeta &mt = meta[size_t(b)];
+    RotMeta mt{};
     mt.rot_off = int64_t(nrot);
-    mt.sw_off = int64_t(nsw);
+    mt.nrot = int32_t(o.rots.size());
     mt.ev_off = int64_t(nev);
     mt.a_off = t.a_off;
     mt.w_off = t.w_off;
     mt.z_off = t.z_off;
     mt.n = t.n;
-    mt.nsw = int32_t(o.sw.size());
     nrot += o.rots.size();
-    nsw += o.sw.size();
     nev += o.evr.size();
+    const int k = t.n <= 64 ? 0 : t.n <= 128 ? 1 : t.n <= 256 ? 2 : 3;
+    cls[k].push_back(mt);
+    maxn[k] = std::max(maxn[k], t.n);
   }
-  const size_t mb = sizeof(WaveMeta) * meta.size(), sb = sizeof(SweepRec) * nsw, rb = sizeof(double2) * nrot,
-               eb = sizeof(double) * nev;
-  const size_t o_s = (mb + 255) & ~size_t(255), o_r = (o_s + sb + 255) & ~size_t(255),
-               o_e = (o_r + rb + 255) & ~size_t(255);
+  std::vector<RotMeta> meta;
+  for (auto &c : cls) meta.insert(meta.end(), c.begin(), c.end());
+  const size_t mb = sizeof(RotMeta) * meta.size(), rb = sizeof(double2) * nrot, eb = sizeof(double) * nev;
+  const size_t o_r = (mb + 255) & ~size_t(255), o_e = (o_r + rb + 255) & ~size_t(255);
   std::vector<char> host(o_e + eb);
   std::memcpy(host.data(), meta.data(), mb);
   {
-    size_t a = 0, c = 0, v = 0;
+    size_t c = 0, v = 0;
     for (const QlOut &o : outs) {
-      std::memcpy(host.data() + o_s + a, o.sw.data(), sizeof(SweepRec) * o.sw.size());
       std::memcpy(host.data() + o_r + c, o.rots.data(), sizeof(double2) * o.rots.size());
       std::memcpy(host.data() + o_e + v, o.evr.data(), sizeof(double) * o.evr.size());
-      a += sizeof(SweepRec) * o.sw.size();
       c += sizeof(double2) * o.rots.size();
       v += sizeof(double) * o.evr.size();
     }
@@ -1312,10 +1328,19 @@ int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, do
   char *u = static_cast<char *>(ub);
   rc = stage_upload(ctx, u, host.data(), host.size());
   if (!rc) {
-    hipLaunchKernelGGL(k_syev_wave, dim3(unsigned(ntask), unsigned((maxn + WT - 1) / WT)), dim3(WT), 0, ctx->stream,
-                       reinterpret_cast<const WaveMeta *>(u), reinterpret_cast<const SweepRec *>(u + o_s),
-                       reinterpret_cast<const double2 *>(u + o_r), reinterpret_cast<const double *>(u + o_e), Abase,
-                       Wbase, Z);
+    const RotMeta *dm = reinterpret_cast<const RotMeta *>(u);
+    const double2 *dr = reinterpret_cast<const double2 *>(u + o_r);
+    const double *dv = reinterpret_cast<const double *>(u + o_e);
+    using Kfn = void (*)(const RotMeta *, const double2 *, const double *, double *, double *, double *);
+    const Kfn fns[4] = {k_syev_apply<64>, k_syev_apply<128>, k_syev_apply<256>, k_syev_apply<0>};
+    size_t first = 0;
+    for (int k = 0; k < 4; ++k) {
+      const size_t cnt = cls[k].size();
+      if (cnt)
+        hipLaunchKernelGGL(fns[k], dim3(unsigned(cnt), unsigned((maxn[k] + AT - 1) / AT)), dim3(AT), 0, ctx->stream,
+                           dm + first, dr, dv, Abase, Wbase, Z);
+      first += cnt;
+    }
     e = hipGetLastError();
     if (e != hipSuccess) rc = hip_err(ctx, e, "itx_syev apply");
   }
@@ -1328,14 +1353,20 @@ int syev_own(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, do
 // ---- blocks above kOwnMax rows: rocSOLVER's divide-and-conquer dsyevd ----
 // ROCm's LAPACK in place of the reference's dsyev on -M: each block is
 // negated into a slot of its size class (N x N, column-major), whose
-// padding is c I with c = 1 + 2 |A|_F, above every eigenvalue of -A; the
+// padding is c I with c = 1 + 2 n max|a_ij|, above every eigenvalue of -A; the
 // tridiagonal form then splits exactly at the block edge, so the first n
 // ascending eigenpairs of the slot are those of -A: A's eigenvalues in
 // descending order (equal ones in LAPACK's order, as in the reference),
-// their eigenvectors zero outside the block.  One strided-batched call per
-// size class (blocks within a factor 1.26 of the class's largest, at most
-// twice the flops of unpadded blocks).
+// their eigenvectors zero outside the block.  dsyevd is bound by the
+// latency of its serial phases -- measured on MI355X: about 2 ms + 20 us
+// per row for one matrix, and a batch of 8 costs little more than one --
+// so every block of a call goes into as few strided-batched calls as
+// possible: one, padded to the largest block, unless the slots would pass
+// 1 GiB.  When a call holds a block above kOwnMax rows, the blocks above
+// kJoinMin rows join that call (their own-kernel time would add to it);
+// otherwise every block stays on our kernels.
 constexpr int kOwnMax = 256;
+constexpr int kJoinMin = 128;
 constexpr int EMAX_LARGE = 8192;
 
 struct PadTask {
@@ -1343,38 +1374,53 @@ struct PadTask {
   int32_t n, N;
 };
 
-__global__ __launch_bounds__(256) void k_pad_neg(const PadTask *__restrict__ tasks, const double *__restrict__ Abase,
-                                                 double *__restrict__ Pbase) {
-  __shared__ double red[4];
+// Kernels over (block, chunk of PCOL columns): max |a_ij| of each block
+// (an integer maximum of the non-negative bit patterns: order-independent),
+// then the negated block in its slot with c = 1 + 2 n max|a_ij| (above
+// every |eigenvalue|) on the padding's diagonal, then the results back.
+constexpr int PCOL = 16;
+
+__global__ __launch_bounds__(256) void k_block_absmax(const PadTask *__restrict__ tasks,
+                                                      const double *__restrict__ Abase,
+                                                      unsigned long long *__restrict__ maxbits) {
   const PadTask t = tasks[blockIdx.x];
+  const int c0 = blockIdx.y * PCOL;
+  if (c0 >= t.n) return;
   const double *__restrict__ A = Abase + t.a_off;
-  const int64_t nn = int64_t(t.n) * t.n, NN = int64_t(t.N) * t.N;
-  double s = 0;
-  for (int64_t x = threadIdx.x; x < nn; x += 256) s = fma(A[x], A[x], s);
+  double m = 0;
+  for (int c = c0; c < min(t.n, c0 + PCOL); ++c)
+    for (int r = threadIdx.x; r < t.n; r += 256) m = fmax(m, fabs(A[r + int64_t(c) * t.n]));
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  const double c = 1.0 + 2.0 * sqrt(red[0] + red[1] + red[2] + red[3]);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&maxbits[blockIdx.x], (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ __launch_bounds__(256) void k_pad_neg(const PadTask *__restrict__ tasks, const double *__restrict__ Abase,
+                                                 const unsigned long long *__restrict__ maxbits,
+                                                 double *__restrict__ Pbase) {
+  const PadTask t = tasks[blockIdx.x];
+  const int c0 = blockIdx.y * PCOL;
+  if (c0 >= t.N) return;
+  const double c = 1.0 + 2.0 * double(t.n) * __longlong_as_double((long long)maxbits[blockIdx.x]);
+  const double *__restrict__ A = Abase + t.a_off;
   double *__restrict__ P = Pbase + t.p_off;
-  for (int64_t x = threadIdx.x; x < NN; x += 256) {
-    const int64_t col = x / t.N, row = x - col * t.N;
-    P[x] = (row < t.n && col < t.n) ? -A[row + col * t.n] : (row == col ? c : 0.0);
-  }
+  for (int col = c0; col < min(t.N, c0 + PCOL); ++col)
+    for (int r = threadIdx.x; r < t.N; r += 256)
+      P[r + int64_t(col) * t.N] = (r < t.n && col < t.n) ? -A[r + int64_t(col) * t.n] : (r == col ? c : 0.0);
 }
 
 __global__ __launch_bounds__(256) void k_unpad(const PadTask *__restrict__ tasks, const double *__restrict__ Pbase,
                                                const double *__restrict__ Dbase, double *__restrict__ Abase,
                                                double *__restrict__ Wbase) {
   const PadTask t = tasks[blockIdx.x];
+  const int c0 = blockIdx.y * PCOL;
+  if (c0 >= t.n) return;
   const double *__restrict__ P = Pbase + t.p_off;
   double *__restrict__ A = Abase + t.a_off;
-  const int64_t nn = int64_t(t.n) * t.n;
-  for (int64_t x = threadIdx.x; x < nn; x += 256) {
-    const int64_t col = x / t.n, row = x - col * t.n;
-    A[x] = P[row + col * t.N];
-  }
-  for (int k = threadIdx.x; k < t.n; k += 256) Wbase[t.w_off + k] = -Dbase[t.d_off + k];
+  for (int col = c0; col < min(t.n, c0 + PCOL); ++col)
+    for (int r = threadIdx.x; r < t.n; r += 256) A[r + int64_t(col) * t.n] = P[r + int64_t(col) * t.N];
+  if (blockIdx.y == 0)
+    for (int k = threadIdx.x; k < t.n; k += 256) Wbase[t.w_off + k] = -Dbase[t.d_off + k];
 }
 
 struct LargeCall {
@@ -1408,8 +1454,9 @@ int syev_large_launch(itx_ctx *ctx, std::vector<PadTask> tasks, double *Abase, d
   int64_t pbytes = 0, dsz = 0;
   for (int i = 0; i < int(tasks.size());) {
     const int N = (tasks[size_t(i)].n + 31) & ~31;
+    const int per_call = int(std::max<int64_t>(1, (int64_t(1) << 27) / (int64_t(N) * N)));  // 1 GiB of slots
     int j = i;
-    while (j < int(tasks.size()) && double(tasks[size_t(j)].n) * 1.26 >= double(N)) {
+    while (j < int(tasks.size()) && j - i < per_call) {
       tasks[size_t(j)].N = N;
       tasks[size_t(j)].p_off = pbytes + int64_t(j - i) * N * N;
       tasks[size_t(j)].d_off = dsz + int64_t(j - i) * N;
@@ -1426,7 +1473,8 @@ int syev_large_launch(itx_ctx *ctx, std::vector<PadTask> tasks, double *Abase, d
   const size_t o_d = o_p + sizeof(double) * size_t(pbytes);
   const size_t o_e = o_d + sizeof(double) * size_t(dsz);
   const size_t o_i = o_e + sizeof(double) * size_t(dsz);
-  int rc = pool_alloc(ctx, o_i + sizeof(int) * tasks.size(), &lc.ws);
+  const size_t o_m = (o_i + sizeof(int) * tasks.size() + 255) & ~size_t(255);
+  int rc = pool_alloc(ctx, o_m + sizeof(double) * tasks.size(), &lc.ws);
   if (rc) return rc;
   char *w = static_cast<char *>(lc.ws);
   rc = stage_upload(ctx, w, tasks.data(), tb);
@@ -1436,7 +1484,14 @@ int syev_large_launch(itx_ctx *ctx, std::vector<PadTask> tasks, double *Abase, d
          *E = reinterpret_cast<double *>(w + o_e);
   lc.info = reinterpret_cast<int *>(w + o_i);
   lc.count = int(tasks.size());
-  hipLaunchKernelGGL(k_pad_neg, dim3(unsigned(tasks.size())), dim3(256), 0, ctx->stream, dt, Abase, P);
+  unsigned long long *maxbits = reinterpret_cast<unsigned long long *>(w + o_m);
+  const unsigned nchunk = unsigned((tasks.front().N + PCOL - 1) / PCOL);  // the largest slot
+  rc = itx_fill(ctx, reinterpret_cast<double *>(maxbits), tasks.size(), 0.0);  // bit pattern 0
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_block_absmax, dim3(unsigned(tasks.size()), nchunk), dim3(256), 0, ctx->stream, dt, Abase,
+                     maxbits);
+  hipLaunchKernelGGL(k_pad_neg, dim3(unsigned(tasks.size()), nchunk), dim3(256), 0, ctx->stream, dt, Abase, maxbits,
+                     P);
   for (const Cls &c : cls) {
     const PadTask &t0 = tasks[size_t(c.first)];
     const rocblas_status st = rocsolver_dsyevd_strided_batched(
@@ -1444,7 +1499,7 @@ int syev_large_launch(itx_ctx *ctx, std::vector<PadTask> tasks, double *Abase, d
         D + t0.d_off, c.N, E + t0.d_off, c.N, lc.info + c.first, c.count);
     if (st != rocblas_status_success) return set_err(ctx, ITX_ERR_HIP, "itx_syev: rocsolver_dsyevd failed");
   }
-  hipLaunchKernelGGL(k_unpad, dim3(unsigned(tasks.size())), dim3(256), 0, ctx->stream, dt, P, D, Abase, Wbase);
+  hipLaunchKernelGGL(k_unpad, dim3(unsigned(tasks.size()), nchunk), dim3(256), 0, ctx->stream, dt, P, D, Abase, Wbase);
   lc.tasks = std::move(tasks);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? ITX_OK : hip_err(ctx, e, "itx_syev (large blocks)");
@@ -1471,9 +1526,9 @@ int syev_large_finish(itx_ctx *ctx, LargeCall &lc, const double *Wbase, double *
   return ITX_OK;
 }
 
-// Every matrix of a call: blocks up to kOwnMax rows (all of them with
-// ITX_EIG_OWN=1, up to EMAX) through our kernels, larger ones through
-// rocSOLVER, enqueued while the host runs the QL recurrences of the others.
+// Every matrix of a call: small blocks (all of them with ITX_EIG_OWN=1, up
+// to EMAX) through our kernels, the others through rocSOLVER, enqueued
+// while the host runs the QL recurrences of the small ones.
 int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs, double *Abase,
                  const int64_t *woffs, double *Wbase, double *whost) {
   if (nmat <= 0) return ITX_OK;
@@ -1484,13 +1539,16 @@ int syev_batched(itx_ctx *ctx, int nmat, const int32_t *ns, const int64_t *aoffs
   std::vector<int32_t> on;
   std::vector<int64_t> oa, ow;
   std::vector<PadTask> large;
+  bool any_large = false;
+  for (int b = 0; b < nmat; ++b) any_large = any_large || ns[b] > kOwnMax;
+  const int own_max = own_only ? EMAX : any_large ? kJoinMin : kOwnMax;
   for (int b = 0; b < nmat; ++b) {
     const int n = ns[b];
     if (n < 0 || n > (own_only ? EMAX : EMAX_LARGE))
       return set_err(ctx, ITX_ERR_UNSUPPORTED,
                      own_only ? "itx_syev: n above 2048 (ITX_EIG_OWN)" : "itx_syev: n above 8192");
     if (n == 0) continue;
-    if (own_only || n <= kOwnMax) {
+    if (n <= own_max) {
       on.push_back(n);
       oa.push_back(aoffs[b]);
       ow.push_back(woffs[b]);

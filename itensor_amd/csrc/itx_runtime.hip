@@ -87,7 +87,7 @@ int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }  // namespace itx
 
 namespace {
-constexpr size_t kStageBytes = size_t(8) << 20;
+constexpr size_t kStageBytes = size_t(64) << 20;  // plan tables of large QN contractions stay asynchronous
 
 // Device memory of itx_contract_qn_noalloc: one copy per device for the
 // life of the process, shared by every call on that device.  In host code

@@ -192,4 +192,8 @@ def hubbard_cylinder(H, Nx: int, Ny: int, t: float = 1.0, U: float = 4.0, yperio
 
 
 DMRG_MAXDIM = [10, 20, 50, 100, 200, 400, 800, 1200, 1600, 2048]
+# MinDim = MaxDim ramp of the benchmark's large-bond leg: the bond dimension
+# follows it instead of saturating at the cutoff (about 110 on this chain);
+# it stops at 1024 so that the leg fits the benchmark's time budget
+DMRG_BIG_MAXDIM = [10, 20, 50, 100, 200, 400, 600, 800, 1024]
 HUBBARD_MAXDIM = [50, 100, 200, 400, 400, 400, 400, 400]  # ramp, then converged at 400
