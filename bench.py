@@ -40,7 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DMRG sec/sweep + Hψ TFLOP/s, mixed vs FP64 on 1 MI355X at |ΔE|≤1e-9"
 # TFLOP/s dense MFMA (datasheet); the BF16x3 form runs 3 BF16 products per
 # FP64-equivalent product, so its FP64-equivalent ceiling is 2500/3
-PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3, "bf16x3s": 2500.0 / 3}
 SEED = 20261202 + 2
 
 from itensor_amd.workloads import (DMRG_BIG_MAXDIM, DMRG_MAXDIM, FUSED, HUBBARD_MAXDIM, SECTORS_1024, STAGES,  # noqa: E402
@@ -349,40 +349,77 @@ def main():
     inputs = [d_psi] + bufs[:-1]
 
     def run_forms(prec):
-        plans = [ctx.plan_dense(da, la, db, lb, dc, lc, prec)
-                 for (da, db, dc), (la, lb, lc) in zip(dims, labels)]
-        return plans
+        if prec == "bf16x3s":  # BF16x3 with pre-split operands on the MFMA stages
+            plans = [ctx.plan_dense(da, la, db, lb, dc, lc, "bf16x3" if i != 1 else "f64")
+                     for i, ((da, db, dc), (la, lb, lc)) in enumerate(zip(dims, labels))]
+            for i in (0, len(plans) - 1):
+                plans[i].set_split(True)
+            return plans
+        return [ctx.plan_dense(da, la, db, lb, dc, lc, prec) for (da, db, dc), (la, lb, lc) in zip(dims, labels)]
 
-    def hpsi(plans):
-        for p, a, b, c in zip(plans, inputs, ops, bufs):
-            p.execute(a.data_ptr(), b.data_ptr(), c.data_ptr())
+    # Pre-split form (fused chain): L and R are split once, as the
+    # environments of a Davidson call are; psi and the MPO stage's output are
+    # split inside every step (their passes timed as their own stages).
+    split_form = args.chain == "fused"
+    if split_form:
+        def words(n):
+            return torch.empty(n, dtype=torch.int32, device=dev)
+
+        w_in = [words(d_psi.numel()), None, words(bufs[1].numel())]
+        w_op = [words(d_L.numel()), None, words(d_R.numel())]
+        ctx.split_bf16(d_L.data_ptr(), w_op[0].data_ptr(), d_L.numel())
+        ctx.split_bf16(d_R.data_ptr(), w_op[2].data_ptr(), d_R.numel())
+
+    def launches(prec, plans):
+        """The step as a list of launches (name, callable)."""
+        if prec != "bf16x3s":
+            return [("S%d" % (i + 1), (lambda p=p, a=a, b=b, c=c: p.execute(a.data_ptr(), b.data_ptr(), c.data_ptr())))
+                    for i, (p, a, b, c) in enumerate(zip(plans, inputs, ops, bufs))]
+        out = []
+        for i, (p, a, b, c) in enumerate(zip(plans, inputs, ops, bufs)):
+            if i == 1:
+                out.append(("S2", lambda p=p, a=a, b=b, c=c: p.execute(a.data_ptr(), b.data_ptr(), c.data_ptr())))
+                continue
+            out.append(("split%d" % (i + 1), lambda a=a, w=w_in[i]: ctx.split_bf16(a.data_ptr(), w.data_ptr(), a.numel())))
+            out.append(("S%d" % (i + 1),
+                        lambda p=p, wa=w_in[i], wb=w_op[i], c=c: p.execute(wa.data_ptr(), wb.data_ptr(), c.data_ptr())))
+        return out
 
     results = {}
     flops = hpsi_flops(chi)
-    for prec in ([args.prec] + [p for p in ("f64", "f32", "bf16x3") if p != args.prec]):
+    forms_run = [args.prec] + [p for p in ("f64", "f32", "bf16x3") if p != args.prec]
+    if split_form:
+        forms_run.append("bf16x3s")
+    for prec in forms_run:
         plans = run_forms(prec)
+        steps = launches(prec, plans)
         for _ in range(args.warmup):
-            hpsi(plans)
+            for _, f in steps:
+                f()
         torch.cuda.synchronize()
         # whole-step timing (wall clock around K steps)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            hpsi(plans)
+            for _, f in steps:
+                f()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        # per-stage HIP-event timing on the stream the kernels run on
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nst + 1)] for _ in range(args.steps)]
+        # per-launch HIP-event timing on the stream the kernels run on
+        nl = len(steps)
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nl + 1)] for _ in range(args.steps)]
         for s in range(args.steps):
-            for i, (p, a, b, c) in enumerate(zip(plans, inputs, ops, bufs)):
+            for i, (_, f) in enumerate(steps):
                 ev[s][i].record(stream)
-                p.execute(a.data_ptr(), b.data_ptr(), c.data_ptr())
-            ev[s][nst].record(stream)
+                f()
+            ev[s][nl].record(stream)
         torch.cuda.synchronize()
-        stage_ms = [float(np.mean([ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)]))
-                    for i in range(nst)]
+        launch_ms = {name: float(np.mean([ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)]))
+                     for i, (name, _) in enumerate(steps)}
+        stage_ms = [launch_ms["S%d" % (i + 1)] for i in range(nst)]
         out = bufs[-1].detach().cpu().numpy().reshape((chi, 2, 2, chi), order="F")
         results[prec] = {"wall_s": wall, "stage_ms": stage_ms, "out": out,
-                         "stage_flops": [p.flops for p in plans]}
+                         "stage_flops": [p.flops for p in plans],
+                         "split_ms": {k: v for k, v in launch_ms.items() if k.startswith("split")}}
         for p in plans:
             p.close()
 
@@ -433,6 +470,10 @@ def main():
                                "roofline": {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK[prec],
                                             "unit": "TFLOP/s (FP64-equivalent)", "frac": round(ach / PEAK[prec], 4),
                                             "stage": dk + 1, "avg_launch_ms": round(sm[dk], 4)}}
+        if r["split_ms"]:
+            line["forms"][prec]["split_pass_ms"] = {k: round(v, 4) for k, v in r["split_ms"].items()}
+            if "bf16x3" in results:
+                line["forms"][prec]["bitwise_equal_to_bf16x3"] = bool(np.array_equal(r["out"], results["bf16x3"]["out"]))
 
     cores, nproc, cpu_model = host_cpu()
     cpu_model = "%s (%d cores visible)" % (cpu_model, nproc)

@@ -175,6 +175,19 @@ int itx_tune_save(itx_ctx *ctx, const char *path);
 int itx_tune_load(itx_ctx *ctx, const char *path);
 int itx_tune_size(itx_ctx *ctx);
 
+/* Pre-split operands of the BF16x3 form (no counterpart in the reference):
+ * itx_split_bf16 writes y[i] = hi | lo << 16, the two BF16 terms of x[i]
+ * rounded to FP32 (hi = bf16(x), lo = bf16(x - hi)) that the BF16x3
+ * kernels otherwise form at every tile load; itx_plan_set_split(plan, 1)
+ * makes a BF16x3 plan whose operands are both contiguous along the
+ * contracted index read such words (A and B passed to itx_plan_execute
+ * then point to the words, n per operand element), so an operand reused
+ * by many products (the environments of a Davidson call) is split once and
+ * streamed at 4 bytes per element.  The results equal the plan's without
+ * pre-splitting bit for bit.  ITX_ERR_UNSUPPORTED for other plans. */
+int itx_split_bf16(itx_ctx *ctx, const double *x, uint32_t *y, size_t n);
+int itx_plan_set_split(itx_ctx *ctx, itx_plan *plan, int on);
+
 /* Algorithmic FP64-equivalent flops of one execution (sum of 2*m*n*k). */
 double itx_plan_flops(const itx_plan *plan);
 /* What a plan runs (diagnostic; no counterpart in the reference): the

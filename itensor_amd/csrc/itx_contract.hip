@@ -26,6 +26,7 @@
 #include <numeric>
 #include <queue>
 #include <sstream>
+#include <type_traits>
 
 #include <unordered_map>
 #include "itx_internal.h"
@@ -312,11 +313,24 @@ __device__ __forceinline__ void split_bf16(double x, unsigned short &hi, unsigne
   lo = (unsigned short)bf16_rne(__float_as_uint(rem));
 }
 
+// The pre-split operand format: y[i] = hi | lo << 16 of split_bf16(x[i]).
+__global__ __launch_bounds__(256) void k_split_bf16(const double *__restrict__ x, uint32_t *__restrict__ y, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    unsigned short h, l;
+    split_bf16(x[i], h, l);
+    y[i] = uint32_t(h) | (uint32_t(l) << 16);
+  }
+}
+
 // With V = 4 (k-contiguous operands whose k runs come in aligned quads,
 // checked when the plan is built) a lane loads four consecutive k of one
 // row as two 16-byte loads and writes their hi and lo terms to LDS as one
 // 8-byte store each.
-template <int BR, bool KC, int NTH, int V = 1>
+// With SP the operand comes pre-split (itx_split_bf16): one 32-bit word per
+// element, hi term in the low half and lo term in the high half, as
+// split_bf16 forms them; the loader moves half the bytes and does no
+// conversion (V = 4: one 16-byte load per quad).
+template <int BR, bool KC, int NTH, int V = 1, bool SP = false>
 struct TileLoader3 {
   static constexpr int PER = BR * BK3 / NTH;
   static_assert(PER >= 1, "tile too small for the thread count");
@@ -324,7 +338,8 @@ struct TileLoader3 {
   static constexpr int NV = PER / V;
   static constexpr int LK = BK3 / V;
   static constexpr int NK = KC ? 1 : PER;
-  double r[PER];
+  using RT = std::conditional_t<SP, uint32_t, double>;
+  RT r[PER];
   bool rok[NV];
   int64_t roff[KC ? NV : 1];
   bool rv[KC ? NV : 1];
@@ -358,12 +373,19 @@ struct TileLoader3 {
       ko[q] = tabK[kv[q] ? k : 0];
     }
   }
-  __device__ __forceinline__ void load(const double *__restrict__ base) {
+  __device__ __forceinline__ void load(const void *__restrict__ base_) {
+    const RT *__restrict__ base = static_cast<const RT *>(base_);
 #pragma unroll
     for (int p = 0; p < NV; ++p) {
       const bool ok = KC ? (kv[0] && rv[p]) : (kv[p] && rv[0]);
       const int64_t off = (KC ? roff[p] + ko[0] : roff[0] + ko[p]);
-      if constexpr (V == 4) {
+      if constexpr (V == 4 && SP) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(base + (ok ? off : 0));
+        r[4 * p] = v.x;
+        r[4 * p + 1] = v.y;
+        r[4 * p + 2] = v.z;
+        r[4 * p + 3] = v.w;
+      } else if constexpr (V == 4) {
         const double2 *q = reinterpret_cast<const double2 *>(base + (ok ? off : 0));
         const double2 v0 = q[0], v1 = q[1];
         r[4 * p] = v0.x;
@@ -377,7 +399,25 @@ struct TileLoader3 {
     }
   }
   __device__ __forceinline__ void store(unsigned short *__restrict__ sh, unsigned short *__restrict__ sl) const {
-    if constexpr (V == 4) {
+    if constexpr (V == 4 && SP) {
+#pragma unroll
+      for (int p = 0; p < NV; ++p) {
+        const int idx = (lr + (NTH / LK) * p) * LDK3 + lk;
+        const uint32_t z = rok[p] ? 0xFFFFFFFFu : 0u;
+        const uint32_t w0 = r[4 * p] & z, w1 = r[4 * p + 1] & z, w2 = r[4 * p + 2] & z, w3 = r[4 * p + 3] & z;
+        *reinterpret_cast<uint2 *>(&sh[idx]) = uint2{(w0 & 0xFFFFu) | (w1 << 16), (w2 & 0xFFFFu) | (w3 << 16)};
+        *reinterpret_cast<uint2 *>(&sl[idx]) =
+            uint2{(w0 >> 16) | (w1 & 0xFFFF0000u), (w2 >> 16) | (w3 & 0xFFFF0000u)};
+      }
+    } else if constexpr (SP) {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        const int idx = KC ? (lr + (NTH / BK3) * p) * LDK3 + lk : lr * LDK3 + lk + (NTH / BR) * p;
+        const uint32_t w = rok[p] ? r[p] : 0u;
+        sh[idx] = (unsigned short)(w & 0xFFFFu);
+        sl[idx] = (unsigned short)(w >> 16);
+      }
+    } else if constexpr (V == 4) {
 #pragma unroll
       for (int p = 0; p < NV; ++p) {
         const int idx = (lr + (NTH / LK) * p) * LDK3 + lk;
@@ -404,15 +444,17 @@ struct TileLoader3 {
   }
 };
 
-template <int BM, int BN, int GM, int GN, bool AKC, bool BKC, int V>
+// SP: both operands pre-split (their pointers are the split words, typed
+// as the other forms' FP64 operands)
+template <int BM, int BN, int GM, int GN, bool AKC, bool BKC, int V, bool SP = false>
 __global__ __launch_bounds__(64 * GM * GN) void k_gemm3(const double *__restrict__ A, const double *__restrict__ B,
                                                         double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                         const WorkUnit *__restrict__ units, int nunits,
                                                         const int64_t *__restrict__ tab, double alpha, double beta) {
   constexpr int NTH = 64 * GM * GN;
   typedef float v4 __attribute__((ext_vector_type(4)));
-  using LA = TileLoader3<BM, AKC, NTH, V>;
-  using LB = TileLoader3<BN, BKC, NTH, V>;
+  using LA = TileLoader3<BM, AKC, NTH, V, SP>;
+  using LB = TileLoader3<BN, BKC, NTH, V, SP>;
   constexpr int WM = BM / GM, WN = BN / GN, FM = WM / 16, FN = WN / 16;
   static_assert(FM >= 1 && FN >= 1, "wave tile below one MFMA fragment");
   __shared__ __align__(16) unsigned short Ah[2][BM * LDK3], Al[2][BM * LDK3];
@@ -449,8 +491,10 @@ __global__ __launch_bounds__(64 * GM * GN) void k_gemm3(const double *__restrict
       LB lb;
       la.init(tid, wu.m0, tk.M, tab + tk.tAM);
       lb.init(tid, wu.n0, tk.N, tab + tk.tBN);
-      const double *Ab = A + tk.a_off;
-      const double *Bb = B + tk.b_off;
+      const void *Ab = SP ? static_cast<const void *>(reinterpret_cast<const uint32_t *>(A) + tk.a_off)
+                          : static_cast<const void *>(A + tk.a_off);
+      const void *Bb = SP ? static_cast<const void *>(reinterpret_cast<const uint32_t *>(B) + tk.b_off)
+                          : static_cast<const void *>(B + tk.b_off);
       const int64_t *tAK = tab + tk.tAK;
       const int64_t *tBK = tab + tk.tBK;
       la.fetch_k(0, tk.K, tAK);
@@ -1101,7 +1145,10 @@ const void *gemm_fn(bool akc, bool bkc, bool vec) {
 }
 
 template <int BM, int BN, int GM, int GN>
-const void *gemm3_fn(bool akc, bool bkc, bool vec) {
+const void *gemm3_fn(bool akc, bool bkc, bool vec, bool split = false) {
+  if (split)  // pre-split operands: k-contiguous layouts only
+    return vec ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true, 4, true>)
+               : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true, 1, true>);
   if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true, 4>);
   if (akc)
     return bkc ? reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, true, true, 1>)
@@ -1110,20 +1157,20 @@ const void *gemm3_fn(bool akc, bool bkc, bool vec) {
              : reinterpret_cast<const void *>(&k_gemm3<BM, BN, GM, GN, false, false, 1>);
 }
 
-const void *gemm3_fn_tile(int tile, bool akc, bool bkc, bool vec) {
+const void *gemm3_fn_tile(int tile, bool akc, bool bkc, bool vec, bool split) {
   switch (tile) {
-    case 0: return gemm3_fn<128, 128, 2, 2>(akc, bkc, vec);
-    case 1: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec);
-    case 2: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec);
-    case 3: return gemm3_fn<64, 64, 2, 2>(akc, bkc, vec);
-    case 6: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec);
-    case 7: return gemm3_fn<256, 64, 4, 2>(akc, bkc, vec);
-    case 8: return gemm3_fn<64, 256, 2, 4>(akc, bkc, vec);
-    case 9: return gemm3_fn<256, 128, 4, 2>(akc, bkc, vec);
-    case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec);
-    case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec);
-    case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec);
-    default: return gemm3_fn<32, 32, 2, 2>(akc, bkc, vec);
+    case 0: return gemm3_fn<128, 128, 2, 2>(akc, bkc, vec, split);
+    case 1: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec, split);
+    case 2: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec, split);
+    case 3: return gemm3_fn<64, 64, 2, 2>(akc, bkc, vec, split);
+    case 6: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec, split);
+    case 7: return gemm3_fn<256, 64, 4, 2>(akc, bkc, vec, split);
+    case 8: return gemm3_fn<64, 256, 2, 4>(akc, bkc, vec, split);
+    case 9: return gemm3_fn<256, 128, 4, 2>(akc, bkc, vec, split);
+    case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec, split);
+    case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec, split);
+    case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec, split);
+    default: return gemm3_fn<32, 32, 2, 2>(akc, bkc, vec, split);
   }
 }
 
@@ -1146,7 +1193,7 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
 }
 
 // Entry point and workgroup size of a plan's kernel.
-const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int vec = 0) {
+const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int vec = 0, bool split = false) {
   if (tile == kSkinny) {  // HBM-bound: FP64 arithmetic in every form
     *block = SK_ROWS;
     return reinterpret_cast<const void *>(&k_skinny);
@@ -1171,8 +1218,10 @@ const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int ve
                : reinterpret_cast<const void *>(&k_small<false, false>);
   }
   *block = 64 * kTiles[tile].gm * kTiles[tile].gn;
-  // vec: bit 0 = runs of 2 usable (FP64 / FP32), bit 1 = runs of 4 (BF16x3)
-  if (prec == ITX_PREC_BF16X3) return gemm3_fn_tile(tile, akc, bkc, (vec & 2) != 0);
+  // vec: bit 0 = runs of 2 usable (FP64 / FP32), bit 1 = runs of 4 (BF16x3),
+  // bit 2 = runs of 4 on 4-element boundaries (BF16x3 pre-split words)
+  if (prec == ITX_PREC_BF16X3)
+    return gemm3_fn_tile(tile, akc, bkc, split ? (vec & 4) != 0 : (vec & 2) != 0, split);
   return prec == ITX_PREC_F64 ? gemm_fn_tile<double>(tile, akc, bkc, (vec & 1) != 0)
                               : gemm_fn_tile<float>(tile, akc, bkc, (vec & 1) != 0);
 }
@@ -1335,28 +1384,31 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     // offset, every row offset even (16-byte aligned given an aligned base)
     fi.vec = 0;
     if (fi.akc && fi.bkc) {
-      auto runs = [&](int64_t t, int64_t n, int V) {
+      // runs of V consecutive k, each starting at a multiple of `al`
+      auto runs = [&](int64_t t, int64_t n, int V, int al) {
         if (n % V) return false;
         for (int64_t k = 0; k < n; k += V) {
-          if (tab[size_t(t + k)] % 2) return false;
+          if (tab[size_t(t + k)] % al) return false;
           for (int j = 1; j < V; ++j)
             if (tab[size_t(t + k + j)] != tab[size_t(t + k)] + j) return false;
         }
         return true;
       };
-      auto even = [&](int64_t t, int64_t n) {
+      auto aligned = [&](int64_t t, int64_t n, int al) {
         for (int64_t i = 0; i < n; ++i)
-          if (tab[size_t(t + i)] % 2) return false;
+          if (tab[size_t(t + i)] % al) return false;
         return true;
       };
-      if (even(fi.tAM, M) && even(fi.tBN, N)) {
-        if (runs(fi.tAK, K, 2) && runs(fi.tBK, K, 2)) fi.vec |= 1;
-        if (runs(fi.tAK, K, 4) && runs(fi.tBK, K, 4)) fi.vec |= 2;
+      if (aligned(fi.tAM, M, 2) && aligned(fi.tBN, N, 2)) {
+        if (runs(fi.tAK, K, 2, 2) && runs(fi.tBK, K, 2, 2)) fi.vec |= 1;
+        if (runs(fi.tAK, K, 4, 2) && runs(fi.tBK, K, 4, 2)) fi.vec |= 2;
       }
+      if (aligned(fi.tAM, M, 4) && aligned(fi.tBN, N, 4) && runs(fi.tAK, K, 4, 4) && runs(fi.tBK, K, 4, 4))
+        fi.vec |= 4;
     }
     return ITX_OK;
   };
-  int vecbits = 3;  // AND over the tasks
+  int vecbits = 7;  // AND over the tasks
   for (int t = 0; t < ntasks; ++t) {
     const int64_t *dA = blkA + size_t(t) * rA;
     const int64_t *dB = blkB + size_t(t) * rB;
@@ -1397,6 +1449,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     }
     vecbits &= fi.vec;
     if (offs[3 * size_t(t)] % 2 || offs[3 * size_t(t) + 1] % 2) vecbits = 0;
+    if (offs[3 * size_t(t)] % 4 || offs[3 * size_t(t) + 1] % 4) vecbits &= ~4;
     GemmTask g{};
     g.a_off = offs[3 * size_t(t)];
     g.b_off = offs[3 * size_t(t) + 1];
@@ -1728,8 +1781,9 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
 }
 
 const void *plan_fn(const itx_plan *p, const double *A, const double *B) {
-  // the vector-load kernel reads 16-byte pairs: its operands must be
-  // 16-byte aligned (caching-pool blocks are 256-byte aligned)
+  // the vector-load kernel reads 16-byte pairs (or quads of split words):
+  // its operands must be 16-byte aligned (caching-pool blocks are 256-byte
+  // aligned)
   const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
   return (p->vec && aligned) ? p->fn : p->fn_scalar;
 }
@@ -1837,6 +1891,38 @@ int itx_plan_execute(itx_ctx *ctx, const itx_plan *plan, const double *A, const 
 }
 
 double itx_plan_flops(const itx_plan *plan) { return plan ? plan->flops : 0.0; }
+
+int itx_split_bf16(itx_ctx *ctx, const double *x, uint32_t *y, size_t n) {
+  if (!ctx || (n && (!x || !y))) return ITX_ERR_ARG;
+  if (n == 0) return ITX_OK;
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, size_t(ctx->num_cus) * 16));
+  hipLaunchKernelGGL(itx::k_split_bf16, dim3(grid), dim3(256), 0, ctx->stream, x, y, n);
+  ITX_CHECK_HIP(ctx, hipGetLastError());
+  return ITX_OK;
+}
+
+int itx_plan_set_split(itx_ctx *ctx, itx_plan *plan, int on) {
+  if (!ctx || !plan) return ITX_ERR_ARG;
+  const bool want = on != 0;
+  if (want == bool(plan->split)) return ITX_OK;
+  if (want && (plan->prec != ITX_PREC_BF16X3 || plan->tile == itx::kSkinnyTile || plan->tile == 10 ||
+               !plan->a_kcontig || !plan->b_kcontig || plan->nred))
+    return itx::set_err(ctx, ITX_ERR_UNSUPPORTED,
+                        "itx_plan_set_split: pre-split operands need a BF16x3 MFMA plan with both operands "
+                        "contiguous along the contracted index");
+  plan->split = want ? 1 : 0;
+  if (plan->nunits > 0) {
+    const bool akc = plan->a_kcontig, bkc = plan->b_kcontig;
+    if (want) {
+      plan->fn = itx::kernel_of(plan->tile, plan->prec, akc, bkc, &plan->block, plan->vec & 4 ? 4 : 0, true);
+      plan->fn_scalar = itx::kernel_of(plan->tile, plan->prec, akc, bkc, &plan->block, 0, true);
+    } else {
+      plan->fn = itx::kernel_of(plan->tile, plan->prec, akc, bkc, &plan->block, plan->vec);
+      plan->fn_scalar = itx::kernel_of(plan->tile, plan->prec, akc, bkc, &plan->block, 0);
+    }
+  }
+  return ITX_OK;
+}
 
 int itx_plan_info(const itx_plan *plan, int *tile, int *form, int *ntasks, int *nunits) {
   if (!plan) return ITX_ERR_ARG;

@@ -69,6 +69,7 @@ struct itx_plan {
   const void *fn = nullptr;  // kernel entry point and workgroup size
   const void *fn_scalar = nullptr;  // the same tile without vector loads
   int vec = 0;  // fn uses 16-byte operand loads (operands must be 16-byte aligned)
+  int split = 0;  // BF16x3 with pre-split operands (itx_plan_set_split)
   int block = 0;
   // split plans: partial tiles into ws, then a reduction pass over
   // nred reduction units (d_red) writes C

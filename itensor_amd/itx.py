@@ -56,6 +56,8 @@ _PROTOS = {
                                   C.c_int, _i64p, _i64p, _i64p, C.c_double, C.c_int]),
     "itx_contract_qn_noalloc": (C.c_int, [_vp, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p, _vp, C.c_int, _i32p,
                                           C.c_int, _i64p, _i64p, _i64p, C.c_double, C.c_int]),
+    "itx_split_bf16": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "itx_plan_set_split": (C.c_int, [_vp, _vp, C.c_int]),
     "itx_plan_dense": (C.c_int, [_vp, C.c_int, _i64p, _i32p, C.c_int, _i64p, _i32p, C.c_int, _i64p,
                                  _i32p, C.c_int, C.POINTER(_vp)]),
     "itx_plan_qn": (C.c_int, [_vp, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i64p,
@@ -198,6 +200,10 @@ class Context:
             float(alpha), PREC[prec])
         self.check(rc, name)
 
+    def split_bf16(self, x: int, y: int, n: int):
+        """y (n 32-bit words) = the BF16 hi / lo terms of the n doubles at x."""
+        self.check(self.L.itx_split_bf16(self.h, _vp(x), _vp(y), n), "itx_split_bf16")
+
     def plan_dense(self, dimsA, labA, dimsB, labB, dimsC, labC, prec="f64") -> "Plan":
         h = _vp()
         rc = self.L.itx_plan_dense(self.h, len(dimsA), _arr(C.c_int64, dimsA), _arr(C.c_int32, labA),
@@ -328,6 +334,10 @@ class Plan:
                        "itx_plan_info")
         return {"tile": t.value, "form": {v: k for k, v in PREC.items()}[f.value], "ntasks": nt.value,
                 "nunits": nu.value}
+
+    def set_split(self, on: bool = True):
+        """BF16x3 plan reading pre-split operand words (itx_split_bf16)."""
+        self.ctx.check(self.ctx.L.itx_plan_set_split(self.ctx.h, self.h, int(bool(on))), "itx_plan_set_split")
 
     def execute(self, A: int, B: int, Cp: int, alpha=1.0, beta=0.0):
         self.ctx.check(self.ctx.L.itx_plan_execute(self.ctx.h, self.h, _vp(A), _vp(B), _vp(Cp),
