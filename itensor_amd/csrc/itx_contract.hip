@@ -1768,7 +1768,16 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
         for (int64_t q = 0; q < tiles; ++q) costs.push_back(c);
       }
       if (costs.empty()) continue;
-      const int occ = tile_occupancy(ti, prec, akc, bkc);
+      // workgroups resident per CU: for the reduced forms the tile's
+      // occupancy, or fewer when the units do not fill that many slots (a CU
+      // running one workgroup does not run it `occupancy` times slower;
+      // without this the FP32 form picked a 64 x 256 tile for 150 units of
+      // the chi = 1024 QN chain, 86 us against 40 us); the FP64 efficiencies
+      // were calibrated against the plain occupancy
+      const int occ_max = tile_occupancy(ti, prec, akc, bkc);
+      const int occ = prec == ITX_PREC_F64
+                          ? occ_max
+                          : int(std::max<int64_t>(1, std::min<int64_t>(occ_max, (int64_t(costs.size()) + cus - 1) / cus)));
       const double eff = (prec == ITX_PREC_F64) ? ts.eff64 : ts.eff32;
       const double tt = lpt_makespan(costs, int64_t(cus) * occ) * occ / eff;
       if (tt < best_t * 0.97) {
