@@ -197,3 +197,6 @@ DMRG_MAXDIM = [10, 20, 50, 100, 200, 400, 800, 1200, 1600, 2048]
 # it stops at 1024 so that the leg fits the benchmark's time budget
 DMRG_BIG_MAXDIM = [10, 20, 50, 100, 200, 400, 600, 800, 1024]
 HUBBARD_MAXDIM = [50, 100, 200, 400, 400, 400, 400, 400]  # ramp, then converged at 400
+# the benchmark's large-bond Hubbard leg: the maxdim ramp toward 4096, stopped at
+# 2500 to fit the benchmark's time budget
+HUBBARD_BIG_MAXDIM = [50, 100, 200, 400, 800, 1200, 1600, 2000, 2500]

@@ -238,3 +238,13 @@ def test_syev_batched_large_size_classes(ctx, dev):
         assert np.all(np.diff(wn) <= 0), n
         assert np.max(np.abs(U.T @ U - np.eye(n))) < 1e-12, n
         assert np.max(np.abs(U @ np.diag(wn) @ U.T - X)) / scale < 1e-12, n
+
+
+def test_syev_rejects_blocks_above_the_limit(ctx, dev):
+    """Blocks above 8192 rows: ITX_ERR_UNSUPPORTED before any device work
+    (the buffers are never touched)."""
+    from itensor_amd.itx import ItxError
+
+    dA, dw = dev.empty(4), dev.empty(4)
+    with pytest.raises(ItxError, match=r"\(4\).*8192"):
+        ctx.syev(8193, dA.data_ptr(), dw.data_ptr())
