@@ -634,6 +634,90 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny(const double *__restrict__ A
   }
 }
 
+// The skinny path when every task has K <= 8 and N <= 8 (the block-pair
+// tasks of a block-sparse MPO stage: site and MPO-link blocks of a few
+// rows): a unit then sums many tiny tasks, and the per-task load round trip
+// and LDS barriers of k_skinny dominate.  Here the B tiles of as many tasks
+// as fit are staged in LDS at once (rows stacked), and the A elements of
+// SKQ_G tasks are loaded together before any is used.  Same accumulation
+// order as k_skinny (tasks in order, k in order), so the same results.
+constexpr int SKQ_K = 8, SKQ_N = 8, SKQ_G = 4;
+
+__global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__ A, const double *__restrict__ B,
+                                                      double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                      const WorkUnit *__restrict__ units, int nunits,
+                                                      const int64_t *__restrict__ tab, double alpha, double beta) {
+  __shared__ double Bs[SK_K * SKQ_N];
+  const int tid = threadIdx.x;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const WorkUnit wu = units[u];
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int N = t0.N, M = t0.M;
+    const int m = wu.m0 + tid;
+    double acc[SKQ_N];
+#pragma unroll
+    for (int n = 0; n < SKQ_N; ++n) acc[n] = 0.0;
+    for (int tc = wu.tbeg; tc < wu.tend;) {
+      // the next tasks whose stacked B rows fit the LDS tile
+      int te = tc, kt = 0;
+      while (te < wu.tend && kt + tasks[te].K <= SK_K) kt += tasks[te++].K;
+      __syncthreads();  // the previous chunk's B rows are no longer read
+      for (int t = tc, kb = 0; t < te; ++t) {
+        const GemmTask tk = tasks[t];
+        for (int x = tid; x < tk.K * N; x += SK_ROWS) {
+          const int k = x / N, n = x - k * N;
+          Bs[(kb + k) * SKQ_N + n] = B[tk.b_off + tab[tk.tBK + k] + tab[tk.tBN + n]];
+        }
+        kb += tk.K;
+      }
+      __syncthreads();
+      if (m < M) {
+        for (int t = tc, kb = 0; t < te; t += SKQ_G) {
+          double av[SKQ_G][SKQ_K];
+          int kk[SKQ_G];
+#pragma unroll
+          for (int q = 0; q < SKQ_G; ++q) {
+            kk[q] = 0;
+            if (t + q < te) {
+              const GemmTask tk = tasks[t + q];
+              kk[q] = tk.K;
+              const double *Ar = A + tk.a_off + tab[tk.tAM + m];
+              const int64_t *tAK = tab + tk.tAK;
+#pragma unroll
+              for (int j = 0; j < SKQ_K; ++j) av[q][j] = (j < tk.K) ? Ar[tAK[j]] : 0.0;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < SKQ_G; ++q) {
+#pragma unroll
+            for (int j = 0; j < SKQ_K; ++j) {
+              if (j < kk[q]) {
+#pragma unroll
+                for (int n = 0; n < SKQ_N; ++n)
+                  if (n < N) acc[n] += av[q][j] * Bs[(kb + j) * SKQ_N + n];
+              }
+            }
+            kb += kk[q];
+          }
+        }
+      }
+      tc = te;
+    }
+    if (m < M) {
+      double *Cm = C + t0.c_off + tab[t0.tCM + m];
+      const int64_t *tCN = tab + t0.tCN;
+#pragma unroll
+      for (int n = 0; n < SKQ_N; ++n) {
+        if (n < N) {
+          double v = alpha * acc[n];
+          if (beta != 0.0) v += beta * Cm[tCN[n]];
+          Cm[tCN[n]] = v;
+        }
+      }
+    }
+  }
+}
+
 // Small-block contractions (C blocks of a few thousand elements: the QN
 // blocks of DMRG at moderate bond dimension): latency-bound, so FP64 VALU
 // arithmetic in every form.  A unit is a 64 x 32 tile of one C block; for
@@ -1193,10 +1277,12 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
 }
 
 // Entry point and workgroup size of a plan's kernel.
-const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int vec = 0, bool split = false) {
+const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int vec = 0, bool split = false,
+                      bool tiny = false) {
   if (tile == kSkinny) {  // HBM-bound: FP64 arithmetic in every form
     *block = SK_ROWS;
-    return reinterpret_cast<const void *>(&k_skinny);
+    // tiny: every task has K <= 8 and N <= 8
+    return tiny ? reinterpret_cast<const void *>(&k_skinny_q) : reinterpret_cast<const void *>(&k_skinny);
   }
   if (tile == kSmall) {  // FP64 on the VALU, FP32 / BF16x3 on the matrix cores
     *block = SM_T;
@@ -1629,8 +1715,9 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->flops = flops;
     p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
     p->vec = (akc && bkc) ? vecbits : 0;
-    p->fn = kernel_of(best, prec, akc, bkc, &p->block, p->vec);
-    p->fn_scalar = kernel_of(best, prec, akc, bkc, &p->block, 0);
+    const bool tiny = best == kSkinny && maxN <= SKQ_N && maxK <= SKQ_K;
+    p->fn = kernel_of(best, prec, akc, bkc, &p->block, p->vec, false, tiny);
+    p->fn_scalar = kernel_of(best, prec, akc, bkc, &p->block, 0, false, tiny);
     p->nred = int(red.size());
     if (p->nred) {
       p->fn_red = reinterpret_cast<const void *>(&k_reduce_tiles);
