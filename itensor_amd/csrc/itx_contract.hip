@@ -289,6 +289,293 @@ __global__ __launch_bounds__(64 * GM * GN) void k_gemm(const double *__restrict_
   }
 }
 
+// ---- FP64 register-pipelined tile ----------------------------------------
+// 128 x 128 C tile, four waves of 64 x 64 (4 x 4 fragments of the f64
+// 16x16x4 MFMA), 16-deep k stages, for operands contiguous along k in
+// aligned pairs (the plan's vector-load condition).  A thread loads 16-byte
+// k pairs; an LDS row holds the 16 k of a stage contiguously (pitch 18
+// doubles: the 16 rows that a quarter-wave reads land on distinct 16-byte
+// bank groups), and a lane reads the 4 consecutive k of its fragment row
+// with two 16-byte reads.  MFMA s (0..3) of a stage takes k = 4q + s from
+// the lanes of quarter q, so every k of the stage enters once.
+// Pipeline (one barrier per stage): while the 64 MFMAs of stage n run from
+// registers, the operands of stage n+1 are read from LDS, stage n+2 is
+// stored to LDS from the load registers, and the global loads of stage n+4
+// are issued into those registers (two register sets: loads have two
+// stages, about 8000 MFMA cycles, to arrive).
+constexpr int DG_BK = 16, DG_P = 18, DG_NT = 256;
+
+struct DgRegs {  // one stage of one operand in flight: four 16-byte pairs
+  double g[8];
+  bool ok[4];
+};
+
+struct DgLoader {  // one operand: 128 rows x 16 k per stage, four 16-byte loads per thread
+  int64_t roff[4];
+  bool rv[4];
+  int64_t ko[2];  // k offsets of the next two stages to load (slot = stage & 1)
+  bool kv[2];
+  int lr, lc;  // base row (32 rows apart per load), k pair
+  __device__ __forceinline__ void init(int tid, int row0, int R, const int64_t *__restrict__ tabR) {
+    lc = tid & 7;
+    lr = tid >> 3;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int row = row0 + lr + 32 * p;
+      rv[p] = row < R;
+      roff[p] = tabR[rv[p] ? row : 0];
+    }
+  }
+  __device__ __forceinline__ void fetch_k(int k0, int K, const int64_t *__restrict__ tabK, int slot) {
+    const int k = k0 + 2 * lc;  // K is even: the pair is in range with its first k
+    kv[slot] = k < K;
+    ko[slot] = tabK[kv[slot] ? k : 0];
+  }
+  __device__ __forceinline__ void load(const double *__restrict__ base, DgRegs &r, int slot) const {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      r.ok[p] = kv[slot] && rv[p];
+      const double2 v = *reinterpret_cast<const double2 *>(base + (r.ok[p] ? roff[p] + ko[slot] : 0));
+      r.g[2 * p] = v.x;
+      r.g[2 * p + 1] = v.y;
+    }
+  }
+  __device__ __forceinline__ void store(double *__restrict__ s, const DgRegs &r) const {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      double2 v;
+      v.x = r.ok[p] ? r.g[2 * p] : 0.0;
+      v.y = r.ok[p] ? r.g[2 * p + 1] : 0.0;
+      *reinterpret_cast<double2 *>(s + (lr + 32 * p) * DG_P + 2 * lc) = v;
+    }
+  }
+};
+
+struct DgFrag {  // a lane's operands of one stage: 4 fragments x 4 k, for A and for B
+  double2 a[4][2], b[4][2];
+};
+
+__device__ __forceinline__ void dg_read(DgFrag &f, const double *__restrict__ as, const double *__restrict__ bs,
+                                        int wm, int wn, int lane) {
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double *pa = as + (wm * 64 + j * 16 + r) * DG_P + 4 * q;
+    f.a[j][0] = *reinterpret_cast<const double2 *>(pa);
+    f.a[j][1] = *reinterpret_cast<const double2 *>(pa + 2);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double *pb = bs + (wn * 64 + i * 16 + r) * DG_P + 4 * q;
+    f.b[i][0] = *reinterpret_cast<const double2 *>(pb);
+    f.b[i][1] = *reinterpret_cast<const double2 *>(pb + 2);
+  }
+}
+
+using dv4 = MF<double>::v4;
+
+__device__ __forceinline__ double dg_k(const double2 (&x)[2], int s) {
+  return s == 0 ? x[0].x : s == 1 ? x[0].y : s == 2 ? x[1].x : x[1].y;
+}
+
+// the 16 MFMAs of k slot s (k = 4q + s in quarter q)
+__device__ __forceinline__ void dg_mfma(dv4 (&acc)[4][4], const DgFrag &f, int s) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = MF<double>::mma(dg_k(f.b[i], s), dg_k(f.a[j], s), acc[i][j]);
+}
+
+__global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, const double *__restrict__ B,
+                                                 double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                 const WorkUnit *__restrict__ units, int nunits,
+                                                 const int64_t *__restrict__ tab, double alpha, double beta) {
+  __shared__ __align__(16) double As[2][128 * DG_P];
+  __shared__ __align__(16) double Bs[2][128 * DG_P];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // XCD-contiguous unit chunks (as in k_gemm)
+  const int G = gridDim.x, b = blockIdx.x;
+  int ustart, uend, ustep;
+  if (G % 8 == 0 && nunits >= G) {
+    const int nb = G / 8, x = b % 8, lb = b / 8;
+    const int chunk = (nunits + 7) / 8;
+    ustart = x * chunk + lb;
+    uend = min(nunits, (x + 1) * chunk);
+    ustep = nb;
+  } else {
+    ustart = b;
+    uend = nunits;
+    ustep = G;
+  }
+
+  for (int u = ustart; u < uend; u += ustep) {
+    const WorkUnit wu = units[u];
+    dv4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0, 0, 0, 0};
+
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      if (tk.K <= 0) continue;
+      DgLoader la, lb;
+      la.init(tid, wu.m0, tk.M, tab + tk.tAM);
+      lb.init(tid, wu.n0, tk.N, tab + tk.tBN);
+      const double *Ab = A + tk.a_off;
+      const double *Bb = B + tk.b_off;
+      const int64_t *tAK = tab + tk.tAK;
+      const int64_t *tBK = tab + tk.tBK;
+      const int S = (tk.K + DG_BK - 1) / DG_BK;
+      DgRegs ra0, rb0, ra1, rb1;  // load register sets
+// load stage st (its k offsets fetched two loads ago, slot st & 1) and
+// fetch the k offsets of stage st + 2 into the same slot: a table read then
+// never waits behind the data loads just issued
+#define ITX_DG_LOAD(st, ra, rb, slot)                      \
+  {                                                        \
+    la.load(Ab, ra, slot);                                 \
+    lb.load(Bb, rb, slot);                                 \
+    la.fetch_k(((st) + 2) * DG_BK, tk.K, tAK, slot);       \
+    lb.fetch_k(((st) + 2) * DG_BK, tk.K, tBK, slot);       \
+  }
+      // prologue: stages 0 and 1 into LDS, stage 0 into registers, stage 2
+      // in set 1 and stage 3 in set 0
+      la.fetch_k(0, tk.K, tAK, 0);
+      lb.fetch_k(0, tk.K, tBK, 0);
+      la.fetch_k(DG_BK, tk.K, tAK, 1);
+      lb.fetch_k(DG_BK, tk.K, tBK, 1);
+      ITX_DG_LOAD(0, ra0, rb0, 0)
+      __syncthreads();  // the previous task's LDS reads are done
+      la.store(As[0], ra0);
+      lb.store(Bs[0], rb0);
+      if (S > 1) ITX_DG_LOAD(1, ra0, rb0, 1)
+      if (S > 2) ITX_DG_LOAD(2, ra1, rb1, 0)
+      __syncthreads();
+      DgFrag f0, f1;
+      dg_read(f0, As[0], Bs[0], wm, wn, lane);
+      if (S > 1) {
+        la.store(As[1], ra0);
+        lb.store(Bs[1], rb0);
+      }
+      if (S > 3) ITX_DG_LOAD(3, ra0, rb0, 1)
+      __syncthreads();
+      // stage n: its MFMAs from registers; stage n+1 read from LDS[(n+1)&1]
+      // at the start; after the first 16 MFMAs, stage n+2 stored from its
+      // register set into LDS[n&1] (read into registers one stage ago) and
+      // the loads of stage n+4 issued into that set.  The scheduling
+      // barriers keep the store and the loads inside the MFMA stream and the
+      // workgroup barrier behind the last MFMA (the stores have long
+      // completed when the waves meet there).
+#define ITX_DG_STEP(n, cur, nxt, ra, rb, slot)                                        \
+  {                                                                               \
+    if ((n) + 1 < S) dg_read(nxt, As[((n) + 1) & 1], Bs[((n) + 1) & 1], wm, wn, lane); \
+    dg_mfma(acc, cur, 0);                                                         \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    if ((n) + 2 < S) {                                                            \
+      la.store(As[(n) & 1], ra);                                                  \
+      lb.store(Bs[(n) & 1], rb);                                                  \
+    }                                                                             \
+    if ((n) + 4 < S) ITX_DG_LOAD((n) + 4, ra, rb, slot)                           \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    dg_mfma(acc, cur, 1);                                                         \
+    dg_mfma(acc, cur, 2);                                                         \
+    dg_mfma(acc, cur, 3);                                                         \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    __syncthreads();                                                              \
+  }
+      // Steady state (stages n with n+4 < S): the same work without
+      // conditions, in one scheduling region, with the LDS reads, the LDS
+      // stores (and their selects) and the global loads (and their address
+      // arithmetic) placed between consecutive MFMAs, so that the MFMA pipe
+      // never waits on the wave's other instructions.
+#define ITX_DG_BODY(n, cur, nxt, ra, rb, id)                                      \
+  {                                                                               \
+    dg_read(nxt, As[((n) + 1) & 1], Bs[((n) + 1) & 1], wm, wn, lane);            \
+    dg_mfma(acc, cur, 0);                                                         \
+    for (int g_ = 0; g_ < 16; ++g_) {                                             \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, id);                         \
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, id);                         \
+    }                                                                             \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    la.store(As[(n) & 1], ra);                                                    \
+    lb.store(Bs[(n) & 1], rb);                                                    \
+    dg_mfma(acc, cur, 1);                                                         \
+    for (int g_ = 0; g_ < 8; ++g_) {                                              \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, id);                         \
+      __builtin_amdgcn_sched_group_barrier(0x006, 4, id);                         \
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, id);                         \
+    }                                                                             \
+    __builtin_amdgcn_sched_group_barrier(0x008, 8, id);                           \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    ITX_DG_LOAD((n) + 4, ra, rb, id)                                              \
+    dg_mfma(acc, cur, 2);                                                         \
+    for (int g_ = 0; g_ < 12; ++g_) {                                             \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, id);                         \
+      __builtin_amdgcn_sched_group_barrier(0x006, 6, id);                         \
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, id);                         \
+    }                                                                             \
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, id);                           \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    dg_mfma(acc, cur, 3);                                                         \
+    __builtin_amdgcn_sched_barrier(0);                                            \
+    __syncthreads();                                                              \
+  }
+      int n = 0;
+      for (; n + 5 < S; n += 2) {
+        ITX_DG_BODY(n, f0, f1, ra1, rb1, 0)
+        ITX_DG_BODY(n + 1, f1, f0, ra0, rb0, 1)
+      }
+      for (; n + 1 < S; n += 2) {
+        ITX_DG_STEP(n, f0, f1, ra1, rb1, 0)
+        ITX_DG_STEP(n + 1, f1, f0, ra0, rb0, 1)
+      }
+      if (n < S) ITX_DG_STEP(n, f0, f1, ra1, rb1, 0)
+#undef ITX_DG_BODY
+#undef ITX_DG_STEP
+#undef ITX_DG_LOAD
+    }
+
+    // epilogue: C = alpha*acc + beta*C through the C offset tables (the
+    // table entries of all of the lane's rows and columns are read before
+    // the first store, from clamped indices)
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int64_t *tCM = tab + t0.tCM;
+    const int64_t *tCN = tab + t0.tCN;
+    double *Cb = C + t0.c_off;
+    int64_t cm[4], cn[4][4];
+    bool mv[4], nv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int m = wu.m0 + wm * 64 + j * 16 + (lane & 15);
+      mv[j] = m < t0.M;
+      cm[j] = tCM[mv[j] ? m : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = wu.n0 + wn * 64 + i * 16 + MF<double>::orow(lane, r);
+        nv[i][r] = n < t0.N;
+        cn[i][r] = tCN[nv[i][r] ? n : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (mv[j] && nv[i][r]) {
+            const int64_t ad = cm[j] + cn[i][r];
+            double v = alpha * acc[i][j][r];
+            if (beta != 0.0) v += beta * Cb[ad];
+            Cb[ad] = v;
+          }
+        }
+  }
+}
+
 // ---- BF16x3 form ----------------------------------------------------------
 // Each FP64 operand value is rounded to FP32 and split into two BF16 terms
 // x ~ hi + lo (hi = bf16(x), lo = bf16(x - hi): 16-17 significant bits);
@@ -1207,8 +1494,10 @@ constexpr TileCfg kTiles[] = {
     {128, 128, 2, 4, 0.80, 0.85},  // 11: as 6 with 32-deep k stages
     {128, 64, 2, 2, 0.90, 0.95},   // 12: as 1 with 32-deep k stages
     {64, 128, 2, 2, 0.90, 0.95},   // 13: as 2 with 32-deep k stages
+    {128, 128, 2, 2, 1.15, 0.85},  // 14: FP64 register-pipelined (k_dgemm), k-contiguous pairs only
 };
-constexpr int kNumTiles = 14;
+constexpr int kNumTiles = 15;
+constexpr int kDgemm = 14;
 constexpr int kSkinny = 5;
 static_assert(kSkinny == kSkinnyTile, "tile index of the skinny path");
 constexpr int kSmall = 10;
@@ -1254,6 +1543,7 @@ const void *gemm3_fn_tile(int tile, bool akc, bool bkc, bool vec, bool split) {
     case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec, split);
     case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec, split);
     case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec, split);
+    case 14: return gemm3_fn<128, 128, 2, 2>(akc, bkc, vec, split);  // not chosen for BF16x3
     default: return gemm3_fn<32, 32, 2, 2>(akc, bkc, vec, split);
   }
 }
@@ -1272,6 +1562,9 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
     case 11: return gemm_fn<T, 128, 128, 2, 4, 32>(akc, bkc, vec);
     case 12: return gemm_fn<T, 128, 64, 2, 2, 32>(akc, bkc, vec);
     case 13: return gemm_fn<T, 64, 128, 2, 2, 32>(akc, bkc, vec);
+    case 14:  // FP64 with aligned k pairs: k_dgemm; otherwise (unaligned operands) tile 0's kernel
+      if (std::is_same<T, double>::value && akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm);
+      return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, false);
     default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc, vec);
   }
 }
@@ -1321,7 +1614,7 @@ int tile_occupancy(int tile, int prec, bool akc, bool bkc) {
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   int block = 0, nb = 0;
-  const void *f = kernel_of(tile, prec, akc, bkc, &block);
+  const void *f = kernel_of(tile, prec, akc, bkc, &block, tile == kDgemm ? 1 : 0);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, 0) != hipSuccess || nb < 1) {
     (void)hipGetLastError();
     nb = 1;
@@ -1621,8 +1914,11 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   for (auto &g : groups) maxKsum = std::max(maxKsum, g.ksum);
 
   // Build the device plan for one tile choice.
+  // the register-pipelined FP64 tile needs k-contiguous operands in aligned pairs
+  const bool dg_ok = prec == ITX_PREC_F64 && akc && bkc && (vecbits & 1);
   auto finalize = [&](int best, itx_plan **pout) -> int {
     *pout = nullptr;
+    if (best == kDgemm && !dg_ok) return ITX_ERR_UNSUPPORTED;
     const int tbm = kTiles[best].bm, tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
     std::vector<WorkUnit> units, red;
     std::vector<double> ucost;
@@ -1801,7 +2097,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     best = kSmall;
   } else if (force) {
     best = std::atoi(force);
-    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow)) best = 0;
+    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow) || (best == kDgemm && !dg_ok)) best = 0;
   } else if (ctx->tune.count(key)) {
     best = ctx->tune[key];
   } else if (ctx->autotune && !into && !st.empty()) {
@@ -1845,8 +2141,8 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     double best_t = 1e300;
     // the 32-deep variants (11-13) are left to measurement (autotuner,
     // ITX_FORCE_TILE): they only pay off for long K on large tiles
-    for (int ti = 0; ti < 11; ++ti) {
-      if (ti == kSkinny || ti == kSmall) continue;
+    for (int ti : {0, 1, 2, 3, 4, 6, 7, 8, 9, kDgemm}) {
+      if (ti == kDgemm && !dg_ok) continue;
       const TileCfg ts = kTiles[ti];
       std::vector<double> costs;
       for (auto &g : groups) {
