@@ -250,6 +250,10 @@ def cpu_dmrg(N: int, maxdims):
     return secs, E
 
 
+# kernels of the engine's tile indices that differ from the generic MFMA tile
+KERNEL_OF_TILE = {5: "k_skinny", 10: "k_small", 14: "k_dgemm"}
+
+
 def measured_traffic(chi: int, prec: str, stage: str):
     """HBM bytes per launch of a stage as rocprofv3 FETCH_SIZE + WRITE_SIZE
     measured them (copy-calibrated) -- read from the committed profile
@@ -426,6 +430,7 @@ def main():
         out = bufs[-1].detach().cpu().numpy().reshape((chi, 2, 2, chi), order="F")
         results[prec] = {"wall_s": wall, "stage_ms": stage_ms, "out": out,
                          "stage_flops": [p.flops for p in plans],
+                         "stage_tiles": [p.info["tile"] for p in plans],
                          "split_ms": {k: v for k, v in launch_ms.items() if k.startswith("split")}}
         for p in plans:
             p.close()
@@ -460,7 +465,9 @@ def main():
                      "traffic": measured_traffic(chi, args.prec, ["S1", "W12", "S4"][dom] if nst == 3 else "none"),
                      "traffic_source": "profiles/pmc_dense_hpsi_chi1024.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
                                        "dispatch, scaled by a 1 GiB copy: x2.0 / x1.0)",
-                     "kernel": "k_gemm (stage %d of %d of H*psi)" % (dom + 1, nst),
+                     "kernel": "%s (stage %d of %d of H*psi, tile %d)" % (
+                         KERNEL_OF_TILE.get(main_r["stage_tiles"][dom], "k_gemm"), dom + 1, nst,
+                         main_r["stage_tiles"][dom]),
                      "flops_per_launch": dom_flops, "avg_launch_ms": round(st[dom], 4)},
         "stages_ms": {args.prec: [round(x, 4) for x in st]},
         "forms": {},
