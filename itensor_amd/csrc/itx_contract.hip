@@ -303,7 +303,22 @@ __global__ __launch_bounds__(64 * GM * GN) void k_gemm(const double *__restrict_
 // stored to LDS from the load registers, and the global loads of stage n+4
 // are issued into those registers (two register sets: loads have two
 // stages, about 8000 MFMA cycles, to arrive).
-constexpr int DG_BK = 16, DG_P = 18, DG_NT = 256;
+// FP32 form: the same pipeline with operands rounded to FP32 at the LDS
+// store, LDS rows of 20 floats (80 bytes: again 16 distinct bank groups),
+// one 16-byte read per fragment row and the f32 16x16x4 MFMA.
+constexpr int DG_BK = 16, DG_NT = 256;
+template <typename T>
+struct DgCfg;
+template <>
+struct DgCfg<double> {
+  static constexpr int P = 18;   // LDS row pitch (elements)
+  static constexpr int RD = 16;  // LDS reads per stage and lane
+};
+template <>
+struct DgCfg<float> {
+  static constexpr int P = 20;
+  static constexpr int RD = 8;
+};
 
 struct DgRegs {  // one stage of one operand in flight: four 16-byte pairs
   double g[8];
@@ -346,30 +361,57 @@ struct DgLoader {  // one operand: 128 rows x 16 k per stage, four 16-byte loads
       double2 v;
       v.x = r.ok[p] ? r.g[2 * p] : 0.0;
       v.y = r.ok[p] ? r.g[2 * p + 1] : 0.0;
-      *reinterpret_cast<double2 *>(s + (lr + 32 * p) * DG_P + 2 * lc) = v;
+      *reinterpret_cast<double2 *>(s + (lr + 32 * p) * DgCfg<double>::P + 2 * lc) = v;
+    }
+  }
+  __device__ __forceinline__ void store(float *__restrict__ s, const DgRegs &r) const {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      float2 v;
+      v.x = r.ok[p] ? float(r.g[2 * p]) : 0.0f;
+      v.y = r.ok[p] ? float(r.g[2 * p + 1]) : 0.0f;
+      *reinterpret_cast<float2 *>(s + (lr + 32 * p) * DgCfg<float>::P + 2 * lc) = v;
     }
   }
 };
 
-struct DgFrag {  // a lane's operands of one stage: 4 fragments x 4 k, for A and for B
+template <typename T>
+struct DgFrag;
+template <>
+struct DgFrag<double> {  // a lane's operands of one stage: 4 fragments x 4 k, for A and for B
   double2 a[4][2], b[4][2];
 };
+template <>
+struct DgFrag<float> {
+  float4 a[4], b[4];
+};
 
-__device__ __forceinline__ void dg_read(DgFrag &f, const double *__restrict__ as, const double *__restrict__ bs,
-                                        int wm, int wn, int lane) {
+__device__ __forceinline__ void dg_read(DgFrag<double> &f, const double *__restrict__ as,
+                                        const double *__restrict__ bs, int wm, int wn, int lane) {
   const int r = lane & 15, q = lane >> 4;
+  constexpr int P = DgCfg<double>::P;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const double *pa = as + (wm * 64 + j * 16 + r) * DG_P + 4 * q;
+    const double *pa = as + (wm * 64 + j * 16 + r) * P + 4 * q;
     f.a[j][0] = *reinterpret_cast<const double2 *>(pa);
     f.a[j][1] = *reinterpret_cast<const double2 *>(pa + 2);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const double *pb = bs + (wn * 64 + i * 16 + r) * DG_P + 4 * q;
+    const double *pb = bs + (wn * 64 + i * 16 + r) * P + 4 * q;
     f.b[i][0] = *reinterpret_cast<const double2 *>(pb);
     f.b[i][1] = *reinterpret_cast<const double2 *>(pb + 2);
   }
+}
+
+__device__ __forceinline__ void dg_read(DgFrag<float> &f, const float *__restrict__ as, const float *__restrict__ bs,
+                                        int wm, int wn, int lane) {
+  const int r = lane & 15, q = lane >> 4;
+  constexpr int P = DgCfg<float>::P;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) f.a[j] = *reinterpret_cast<const float4 *>(as + (wm * 64 + j * 16 + r) * P + 4 * q);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f.b[i] = *reinterpret_cast<const float4 *>(bs + (wn * 64 + i * 16 + r) * P + 4 * q);
 }
 
 using dv4 = MF<double>::v4;
@@ -377,21 +419,28 @@ using dv4 = MF<double>::v4;
 __device__ __forceinline__ double dg_k(const double2 (&x)[2], int s) {
   return s == 0 ? x[0].x : s == 1 ? x[0].y : s == 2 ? x[1].x : x[1].y;
 }
+__device__ __forceinline__ float dg_k(const float4 &x, int s) {
+  return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
+}
 
 // the 16 MFMAs of k slot s (k = 4q + s in quarter q)
-__device__ __forceinline__ void dg_mfma(dv4 (&acc)[4][4], const DgFrag &f, int s) {
+template <typename T>
+__device__ __forceinline__ void dg_mfma(typename MF<T>::v4 (&acc)[4][4], const DgFrag<T> &f, int s) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = MF<double>::mma(dg_k(f.b[i], s), dg_k(f.a[j], s), acc[i][j]);
+    for (int j = 0; j < 4; ++j) acc[i][j] = MF<T>::mma(dg_k(f.b[i], s), dg_k(f.a[j], s), acc[i][j]);
 }
 
+template <typename T>
 __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, const double *__restrict__ B,
                                                  double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                  const WorkUnit *__restrict__ units, int nunits,
                                                  const int64_t *__restrict__ tab, double alpha, double beta) {
-  __shared__ __align__(16) double As[2][128 * DG_P];
-  __shared__ __align__(16) double Bs[2][128 * DG_P];
+  using v4 = typename MF<T>::v4;
+  constexpr int P = DgCfg<T>::P, RD = DgCfg<T>::RD;
+  __shared__ __align__(16) T As[2][128 * P];
+  __shared__ __align__(16) T Bs[2][128 * P];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
@@ -412,11 +461,11 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
 
   for (int u = ustart; u < uend; u += ustep) {
     const WorkUnit wu = units[u];
-    dv4 acc[4][4];
+    v4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0, 0, 0, 0};
+      for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
     for (int t = wu.tbeg; t < wu.tend; ++t) {
       const GemmTask tk = tasks[t];
@@ -453,7 +502,7 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
       if (S > 1) ITX_DG_LOAD(1, ra0, rb0, 1)
       if (S > 2) ITX_DG_LOAD(2, ra1, rb1, 0)
       __syncthreads();
-      DgFrag f0, f1;
+      DgFrag<T> f0, f1;
       dg_read(f0, As[0], Bs[0], wm, wn, lane);
       if (S > 1) {
         la.store(As[1], ra0);
@@ -494,8 +543,8 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
   {                                                                               \
     dg_read(nxt, As[((n) + 1) & 1], Bs[((n) + 1) & 1], wm, wn, lane);            \
     dg_mfma(acc, cur, 0);                                                         \
-    for (int g_ = 0; g_ < 16; ++g_) {                                             \
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, id);                         \
+    for (int g_ = 0; g_ < RD; ++g_) {                                             \
+      __builtin_amdgcn_sched_group_barrier(0x008, 16 / RD, id);                   \
       __builtin_amdgcn_sched_group_barrier(0x100, 1, id);                         \
     }                                                                             \
     __builtin_amdgcn_sched_barrier(0);                                            \
@@ -556,7 +605,7 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = wu.n0 + wn * 64 + i * 16 + MF<double>::orow(lane, r);
+        const int n = wu.n0 + wn * 64 + i * 16 + MF<T>::orow(lane, r);
         nv[i][r] = n < t0.N;
         cn[i][r] = tCN[nv[i][r] ? n : 0];
       }
@@ -568,7 +617,7 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
         for (int r = 0; r < 4; ++r) {
           if (mv[j] && nv[i][r]) {
             const int64_t ad = cm[j] + cn[i][r];
-            double v = alpha * acc[i][j][r];
+            double v = alpha * double(acc[i][j][r]);
             if (beta != 0.0) v += beta * Cb[ad];
             Cb[ad] = v;
           }
@@ -1562,8 +1611,8 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
     case 11: return gemm_fn<T, 128, 128, 2, 4, 32>(akc, bkc, vec);
     case 12: return gemm_fn<T, 128, 64, 2, 2, 32>(akc, bkc, vec);
     case 13: return gemm_fn<T, 64, 128, 2, 2, 32>(akc, bkc, vec);
-    case 14:  // FP64 with aligned k pairs: k_dgemm; otherwise (unaligned operands) tile 0's kernel
-      if (std::is_same<T, double>::value && akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm);
+    case 14:  // aligned k pairs: k_dgemm; otherwise (unaligned operands) tile 0's kernel
+      if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm<T>);
       return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, false);
     default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc, vec);
   }
@@ -1915,7 +1964,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
 
   // Build the device plan for one tile choice.
   // the register-pipelined FP64 tile needs k-contiguous operands in aligned pairs
-  const bool dg_ok = prec == ITX_PREC_F64 && akc && bkc && (vecbits & 1);
+  const bool dg_ok = (prec == ITX_PREC_F64 || prec == ITX_PREC_F32) && akc && bkc && (vecbits & 1);
   auto finalize = [&](int best, itx_plan **pout) -> int {
     *pout = nullptr;
     if (best == kDgemm && !dg_ok) return ITX_ERR_UNSUPPORTED;

@@ -1,12 +1,13 @@
-"""GPU parity of the register-pipelined FP64 tile (k_dgemm, tile 14).
+"""GPU parity of the register-pipelined tile (k_dgemm, tile 14).
 
-The tile serves FP64 plans whose operands are both contiguous along the
-contracted index in aligned pairs.  It is forced (ITX_FORCE_TILE=14) on:
+The tile serves FP64 and FP32-form plans whose operands are both contiguous
+along the contracted index in aligned pairs.  It is forced (ITX_FORCE_TILE=14)
+on:
 
 * plain contractions of every pipeline depth (K = 2 .. 1030: one stage,
   the two- to five-stage prologue/tail paths, and the steady-state loop),
   with M and N off the 128 tile edges, alpha and beta, against the oracle's
-  contract() at 1e-12 relative to max |C|;
+  contract() at 1e-12 relative to max |C| (FP64) and 2e-5 (FP32 form);
 * the S1 and S4 stages of the dense H*psi chain (S4: a two-digit
   contracted index whose digits are apart in the operand);
 * operands whose base is not 16-byte aligned (the plan falls back to its
@@ -42,9 +43,12 @@ SHAPES = [
 ]
 
 
-def _plan_run(ctx, dev, A, labA, B, labB, labC, alpha=1.0, beta=0.0, C0=None, shift=0):
+TOL = {"f64": 1e-12, "f32": 2e-5}
+
+
+def _plan_run(ctx, dev, A, labA, B, labB, labC, alpha=1.0, beta=0.0, C0=None, shift=0, prec="f64"):
     ref = contract(A, labA, B, labB, labC, alpha=alpha, beta=beta, C=C0)
-    plan = ctx.plan_dense(A.shape, labA, B.shape, labB, ref.shape, labC, "f64")
+    plan = ctx.plan_dense(A.shape, labA, B.shape, labB, ref.shape, labC, prec)
     try:
         if shift:
             # operands one element into a buffer: 8-byte, not 16-byte, aligned
@@ -63,23 +67,25 @@ def _plan_run(ctx, dev, A, labA, B, labB, labC, alpha=1.0, beta=0.0, C0=None, sh
     return got, ref, info
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_dgemm_tile_every_depth(ctx, dev, monkeypatch, shape):
+def test_dgemm_tile_every_depth(ctx, dev, monkeypatch, shape, prec):
     monkeypatch.setenv("ITX_FORCE_TILE", str(TILE))
     M, N, K = shape
     rng = np.random.default_rng(9000 + M + N + K)
     A = rng.standard_normal((K, M))
     B = rng.standard_normal((K, N))
-    got, ref, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2))
-    assert info["tile"] == TILE
-    assert relerr(got, ref) < 1e-12, shape
+    got, ref, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), prec=prec)
+    assert info["tile"] == TILE and info["form"] == prec
+    assert relerr(got, ref) < TOL[prec], shape
     C0 = rng.standard_normal((M, N))
-    got, ref, _ = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), 0.7, -0.3, C0)
-    assert relerr(got, ref) < 1e-12, shape
+    got, ref, _ = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), 0.7, -0.3, C0, prec=prec)
+    assert relerr(got, ref) < TOL[prec], shape
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("stage", ["S1", "S4"])
-def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage):
+def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage, prec):
     """The S1 stage (psi x L over one index) and the S4 stage (T3 x L over
     a two-digit index, its digits apart in T3) of the dense H*psi chain at
     chi = 256, with the labels the chain uses."""
@@ -89,12 +95,12 @@ def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage):
     L = rng.standard_normal((chi, 5, chi))
     if stage == "S1":
         psi = rng.standard_normal((chi, 2, 2, chi))
-        got, ref, info = _plan_run(ctx, dev, psi, (-1, 2, 3, 4), L, (-1, 5, 6), (2, 3, 4, 5, 6))
+        got, ref, info = _plan_run(ctx, dev, psi, (-1, 2, 3, 4), L, (-1, 5, 6), (2, 3, 4, 5, 6), prec=prec)
     else:
         T3 = rng.standard_normal((chi, chi, 2, 2, 5))
-        got, ref, info = _plan_run(ctx, dev, T3, (-1, 3, 4, 5, -2), L, (-1, -2, 6), (3, 4, 5, 6))
+        got, ref, info = _plan_run(ctx, dev, T3, (-1, 3, 4, 5, -2), L, (-1, -2, 6), (3, 4, 5, 6), prec=prec)
     assert info["tile"] == TILE
-    assert relerr(got, ref) < 1e-12
+    assert relerr(got, ref) < TOL[prec]
 
 
 @pytest.mark.parametrize("shape", [(129, 257, 48), (200, 64, 96)])
