@@ -915,6 +915,10 @@ __global__ __launch_bounds__(64 * GM * GN) void k_gemm3(const double *__restrict
 // while B (K x N) sits in LDS.  Same task/unit/table format as k_gemm.
 constexpr int SK_ROWS = 256, SK_N = 32, SK_K = 64;
 
+// V4: every task's A is contiguous along k in runs of 4 starting at even
+// offsets (K a multiple of 4): a row's 8 k of a batch are two runs, read as
+// four 16-byte loads instead of eight 8-byte loads.
+template <bool V4>
 __global__ __launch_bounds__(SK_ROWS) void k_skinny(const double *__restrict__ A, const double *__restrict__ B,
                                                     double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                     const WorkUnit *__restrict__ units, int nunits,
@@ -941,8 +945,23 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny(const double *__restrict__ A
         const int64_t *tAK = tab + tk.tAK;
         for (int k0 = 0; k0 < K; k0 += 8) {
           double av[8];  // issue 8 independent loads before using any
+          if constexpr (V4) {
+            const bool two = k0 + 4 < K;  // K % 4 == 0: one or two runs left
+            const double2 *r0 = reinterpret_cast<const double2 *>(Ar + tAK[k0]);
+            const double2 *r1 = reinterpret_cast<const double2 *>(Ar + tAK[two ? k0 + 4 : k0]);
+            const double2 x0 = r0[0], x1 = r0[1], x2 = r1[0], x3 = r1[1];
+            av[0] = x0.x;
+            av[1] = x0.y;
+            av[2] = x1.x;
+            av[3] = x1.y;
+            av[4] = two ? x2.x : 0.0;
+            av[5] = two ? x2.y : 0.0;
+            av[6] = two ? x3.x : 0.0;
+            av[7] = two ? x3.y : 0.0;
+          } else {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) av[j] = (k0 + j < K) ? Ar[tAK[k0 + j]] : 0.0;
+            for (int j = 0; j < 8; ++j) av[j] = (k0 + j < K) ? Ar[tAK[k0 + j]] : 0.0;
+          }
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
             if (k0 + j < K) {
@@ -1623,8 +1642,10 @@ const void *kernel_of(int tile, int prec, bool akc, bool bkc, int *block, int ve
                       bool tiny = false) {
   if (tile == kSkinny) {  // HBM-bound: FP64 arithmetic in every form
     *block = SK_ROWS;
-    // tiny: every task has K <= 8 and N <= 8
-    return tiny ? reinterpret_cast<const void *>(&k_skinny_q) : reinterpret_cast<const void *>(&k_skinny);
+    // tiny: every task has K <= 8 and N <= 8; vec bit 3: A in aligned k runs of 4
+    if (tiny) return reinterpret_cast<const void *>(&k_skinny_q);
+    return (vec & 8) ? reinterpret_cast<const void *>(&k_skinny<true>)
+                     : reinterpret_cast<const void *>(&k_skinny<false>);
   }
   if (tile == kSmall) {  // FP64 on the VALU, FP32 / BF16x3 on the matrix cores
     *block = SM_T;
@@ -1745,7 +1766,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     int64_t tAM, tCM, tBN, tCN, tAK, tBK;
     int64_t am, cm, bn, cn, ak, bk;
     bool akc, bkc, empty;
-    int vec;  // bit 0: k runs of 2 usable by both operands, bit 1: runs of 4
+    int vec;  // bit 0: k runs of 2 usable by both operands, bit 1: runs of 4 (bits 2, 3: below)
   };
   std::vector<DimInfo> infos;
   std::vector<int64_t> ikeys;  // the dims of each entry (rA + rB per entry)
@@ -1811,22 +1832,25 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     // consecutive k one run of V consecutive elements starting at an even
     // offset, every row offset even (16-byte aligned given an aligned base)
     fi.vec = 0;
+    // runs of V consecutive k, each starting at a multiple of `al`
+    auto runs = [&](int64_t t, int64_t n, int V, int al) {
+      if (n % V) return false;
+      for (int64_t k = 0; k < n; k += V) {
+        if (tab[size_t(t + k)] % al) return false;
+        for (int j = 1; j < V; ++j)
+          if (tab[size_t(t + k + j)] != tab[size_t(t + k)] + j) return false;
+      }
+      return true;
+    };
+    auto aligned = [&](int64_t t, int64_t n, int al) {
+      for (int64_t i = 0; i < n; ++i)
+        if (tab[size_t(t + i)] % al) return false;
+      return true;
+    };
+    // bit 3: A alone in runs of 4 at even offsets (the skinny path reads
+    // only A from memory)
+    if (fi.akc && aligned(fi.tAM, M, 2) && runs(fi.tAK, K, 4, 2)) fi.vec |= 8;
     if (fi.akc && fi.bkc) {
-      // runs of V consecutive k, each starting at a multiple of `al`
-      auto runs = [&](int64_t t, int64_t n, int V, int al) {
-        if (n % V) return false;
-        for (int64_t k = 0; k < n; k += V) {
-          if (tab[size_t(t + k)] % al) return false;
-          for (int j = 1; j < V; ++j)
-            if (tab[size_t(t + k + j)] != tab[size_t(t + k)] + j) return false;
-        }
-        return true;
-      };
-      auto aligned = [&](int64_t t, int64_t n, int al) {
-        for (int64_t i = 0; i < n; ++i)
-          if (tab[size_t(t + i)] % al) return false;
-        return true;
-      };
       if (aligned(fi.tAM, M, 2) && aligned(fi.tBN, N, 2)) {
         if (runs(fi.tAK, K, 2, 2) && runs(fi.tBK, K, 2, 2)) fi.vec |= 1;
         if (runs(fi.tAK, K, 4, 2) && runs(fi.tBK, K, 4, 2)) fi.vec |= 2;
@@ -1836,7 +1860,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     }
     return ITX_OK;
   };
-  int vecbits = 7;  // AND over the tasks
+  int vecbits = 15;  // AND over the tasks
   for (int t = 0; t < ntasks; ++t) {
     const int64_t *dA = blkA + size_t(t) * rA;
     const int64_t *dB = blkB + size_t(t) * rB;
@@ -1876,7 +1900,8 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
       bkc = fi.bkc;
     }
     vecbits &= fi.vec;
-    if (offs[3 * size_t(t)] % 2 || offs[3 * size_t(t) + 1] % 2) vecbits = 0;
+    if (offs[3 * size_t(t)] % 2) vecbits = 0;
+    if (offs[3 * size_t(t) + 1] % 2) vecbits &= 8;
     if (offs[3 * size_t(t)] % 4 || offs[3 * size_t(t) + 1] % 4) vecbits &= ~4;
     GemmTask g{};
     g.a_off = offs[3 * size_t(t)];
@@ -2059,7 +2084,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->nunits = int(units.size());
     p->flops = flops;
     p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
-    p->vec = (akc && bkc) ? vecbits : 0;
+    p->vec = (akc && bkc) ? vecbits : akc ? (vecbits & 8) : 0;
     const bool tiny = best == kSkinny && maxN <= SKQ_N && maxK <= SKQ_K;
     p->fn = kernel_of(best, prec, akc, bkc, &p->block, p->vec, false, tiny);
     p->fn_scalar = kernel_of(best, prec, akc, bkc, &p->block, 0, false, tiny);
