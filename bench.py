@@ -251,7 +251,7 @@ def cpu_dmrg(N: int, maxdims):
 
 
 # kernels of the engine's tile indices that differ from the generic MFMA tile
-KERNEL_OF_TILE = {5: "k_skinny", 10: "k_small", 14: "k_dgemm"}
+KERNEL_OF_TILE = {5: "k_skinny", 10: "k_small", 14: "k_dgemm", 15: "k_dgemm2"}
 
 
 def measured_traffic(chi: int, prec: str, stage: str):

@@ -625,6 +625,168 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
   }
 }
 
+// ---- FP64 tile, two workgroups per CU ------------------------------------
+// The tile, LDS layout and k order of k_dgemm (results bit for bit equal),
+// with the register budget of two waves per SIMD (128 accumulator + at most
+// 128 other registers): the operands are read from LDS one k slot ahead
+// (16 MFMAs) instead of one stage ahead, and one set of load registers
+// holds the next stage.  Two resident workgroups per CU cover each other's
+// barrier waits, epilogues and first loads.  One barrier per stage, before
+// the last slot, which reads the next stage's first slot.
+struct DgSlot {  // a lane's operands of one k slot: 4 A and 4 B fragments
+  double a[4], b[4];
+};
+
+__device__ __forceinline__ void dg_read_slot(DgSlot &f, const double *__restrict__ as, const double *__restrict__ bs,
+                                             int wm, int wn, int lane, int s) {
+  const int r = lane & 15, q = lane >> 4;
+  constexpr int P = DgCfg<double>::P;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) f.a[j] = as[(wm * 64 + j * 16 + r) * P + 4 * q + s];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f.b[i] = bs[(wn * 64 + i * 16 + r) * P + 4 * q + s];
+}
+
+__device__ __forceinline__ void dg_mfma_slot(dv4 (&acc)[4][4], const DgSlot &f) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = MF<double>::mma(f.b[i], f.a[j], acc[i][j]);
+}
+
+__global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ A, const double *__restrict__ B,
+                                                     double *__restrict__ C, const GemmTask *__restrict__ tasks,
+                                                     const WorkUnit *__restrict__ units, int nunits,
+                                                     const int64_t *__restrict__ tab, double alpha, double beta) {
+  constexpr int P = DgCfg<double>::P;
+  __shared__ __align__(16) double As[2][128 * P];
+  __shared__ __align__(16) double Bs[2][128 * P];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  const int G = gridDim.x, b = blockIdx.x;
+  int ustart, uend, ustep;
+  if (G % 8 == 0 && nunits >= G) {
+    const int nb = G / 8, x = b % 8, lb = b / 8;
+    const int chunk = (nunits + 7) / 8;
+    ustart = x * chunk + lb;
+    uend = min(nunits, (x + 1) * chunk);
+    ustep = nb;
+  } else {
+    ustart = b;
+    uend = nunits;
+    ustep = G;
+  }
+
+  for (int u = ustart; u < uend; u += ustep) {
+    const WorkUnit wu = units[u];
+    dv4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0, 0, 0, 0};
+
+    for (int t = wu.tbeg; t < wu.tend; ++t) {
+      const GemmTask tk = tasks[t];
+      if (tk.K <= 0) continue;
+      DgLoader la, lb;
+      la.init(tid, wu.m0, tk.M, tab + tk.tAM);
+      lb.init(tid, wu.n0, tk.N, tab + tk.tBN);
+      const double *Ab = A + tk.a_off;
+      const double *Bb = B + tk.b_off;
+      const int64_t *tAK = tab + tk.tAK;
+      const int64_t *tBK = tab + tk.tBK;
+      const int K = tk.K, S = (K + DG_BK - 1) / DG_BK;
+      DgRegs ra, rb;
+// load stage st with the k offsets of slot sl (= st & 1, a constant at each
+// use) and fetch stage st + 2's into the same slot
+#define ITX_DG2_LOAD(st, sl)                                  \
+  {                                                           \
+    la.load(Ab, ra, sl);                                      \
+    lb.load(Bb, rb, sl);                                      \
+    la.fetch_k(((st) + 2) * DG_BK, K, tAK, sl);               \
+    lb.fetch_k(((st) + 2) * DG_BK, K, tBK, sl);               \
+  }
+      la.fetch_k(0, K, tAK, 0);
+      lb.fetch_k(0, K, tBK, 0);
+      la.fetch_k(DG_BK, K, tAK, 1);
+      lb.fetch_k(DG_BK, K, tBK, 1);
+      ITX_DG2_LOAD(0, 0)
+      __syncthreads();  // the previous task's LDS reads are done
+      la.store(As[0], ra);
+      lb.store(Bs[0], rb);
+      if (S > 1) ITX_DG2_LOAD(1, 1)
+      __syncthreads();
+      DgSlot cur, nxt;
+      dg_read_slot(cur, As[0], Bs[0], wm, wn, lane, 0);
+      // stage n (buffer n & 1, load slot sl = n & 1): stage n+1 (in the load
+      // registers) into the other buffer, whose stage n-1 reads ended before
+      // the previous barrier; the loads of stage n+2; the four k slots, each
+      // reading the next slot's operands
+#define ITX_DG2_STAGE(n, sl)                                                      \
+  {                                                                               \
+    const double *as = As[sl], *bs = Bs[sl];                                      \
+    if ((n) + 1 < S) {                                                            \
+      la.store(As[(sl) ^ 1], ra);                                                 \
+      lb.store(Bs[(sl) ^ 1], rb);                                                 \
+    }                                                                             \
+    if ((n) + 2 < S) ITX_DG2_LOAD((n) + 2, sl)                                    \
+    dg_read_slot(nxt, as, bs, wm, wn, lane, 1);                                   \
+    dg_mfma_slot(acc, cur);                                                       \
+    dg_read_slot(cur, as, bs, wm, wn, lane, 2);                                   \
+    dg_mfma_slot(acc, nxt);                                                       \
+    dg_read_slot(nxt, as, bs, wm, wn, lane, 3);                                   \
+    dg_mfma_slot(acc, cur);                                                       \
+    __syncthreads(); /* stage n+1 stored by every wave; stage n's reads done */   \
+    if ((n) + 1 < S) dg_read_slot(cur, As[(sl) ^ 1], Bs[(sl) ^ 1], wm, wn, lane, 0); \
+    dg_mfma_slot(acc, nxt);                                                       \
+  }
+      int n = 0;
+      for (; n + 1 < S; n += 2) {
+        ITX_DG2_STAGE(n, 0)
+        ITX_DG2_STAGE(n + 1, 1)
+      }
+      if (n < S) ITX_DG2_STAGE(n, 0)
+#undef ITX_DG2_STAGE
+#undef ITX_DG2_LOAD
+    }
+
+    const GemmTask t0 = tasks[wu.tbeg];
+    const int64_t *tCM = tab + t0.tCM;
+    const int64_t *tCN = tab + t0.tCN;
+    double *Cb = C + t0.c_off;
+    int64_t cm[4], cn[4][4];
+    bool mv[4], nv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int m = wu.m0 + wm * 64 + j * 16 + (lane & 15);
+      mv[j] = m < t0.M;
+      cm[j] = tCM[mv[j] ? m : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = wu.n0 + wn * 64 + i * 16 + MF<double>::orow(lane, r);
+        nv[i][r] = n < t0.N;
+        cn[i][r] = tCN[nv[i][r] ? n : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (mv[j] && nv[i][r]) {
+            const int64_t ad = cm[j] + cn[i][r];
+            double v = alpha * acc[i][j][r];
+            if (beta != 0.0) v += beta * Cb[ad];
+            Cb[ad] = v;
+          }
+        }
+  }
+}
+
 // ---- BF16x3 form ----------------------------------------------------------
 // Each FP64 operand value is rounded to FP32 and split into two BF16 terms
 // x ~ hi + lo (hi = bf16(x), lo = bf16(x - hi): 16-17 significant bits);
@@ -1562,10 +1724,11 @@ constexpr TileCfg kTiles[] = {
     {128, 128, 2, 4, 0.80, 0.85},  // 11: as 6 with 32-deep k stages
     {128, 64, 2, 2, 0.90, 0.95},   // 12: as 1 with 32-deep k stages
     {64, 128, 2, 2, 0.90, 0.95},   // 13: as 2 with 32-deep k stages
-    {128, 128, 2, 2, 1.15, 0.85},  // 14: FP64 register-pipelined (k_dgemm), k-contiguous pairs only
+    {128, 128, 2, 2, 1.15, 0.85},  // 14: register-pipelined (k_dgemm), k-contiguous pairs only
+    {128, 128, 2, 2, 1.10, 0.85},  // 15: as 14, two workgroups per CU (k_dgemm2), FP64 only
 };
-constexpr int kNumTiles = 15;
-constexpr int kDgemm = 14;
+constexpr int kNumTiles = 16;
+constexpr int kDgemm = 14, kDgemm2 = 15;
 constexpr int kSkinny = 5;
 static_assert(kSkinny == kSkinnyTile, "tile index of the skinny path");
 constexpr int kSmall = 10;
@@ -1611,7 +1774,8 @@ const void *gemm3_fn_tile(int tile, bool akc, bool bkc, bool vec, bool split) {
     case 11: return gemm3_fn<128, 128, 2, 4>(akc, bkc, vec, split);
     case 12: return gemm3_fn<128, 64, 2, 2>(akc, bkc, vec, split);
     case 13: return gemm3_fn<64, 128, 2, 2>(akc, bkc, vec, split);
-    case 14: return gemm3_fn<128, 128, 2, 2>(akc, bkc, vec, split);  // not chosen for BF16x3
+    case 14:
+    case 15: return gemm3_fn<128, 128, 2, 2>(akc, bkc, vec, split);  // not chosen for BF16x3
     default: return gemm3_fn<32, 32, 2, 2>(akc, bkc, vec, split);
   }
 }
@@ -1632,6 +1796,9 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
     case 13: return gemm_fn<T, 64, 128, 2, 2, 32>(akc, bkc, vec);
     case 14:  // aligned k pairs: k_dgemm; otherwise (unaligned operands) tile 0's kernel
       if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm<T>);
+      return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, false);
+    case 15:
+      if (std::is_same<T, double>::value && akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm2);
       return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, false);
     default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc, vec);
   }
@@ -1684,7 +1851,7 @@ int tile_occupancy(int tile, int prec, bool akc, bool bkc) {
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   int block = 0, nb = 0;
-  const void *f = kernel_of(tile, prec, akc, bkc, &block, tile == kDgemm ? 1 : 0);
+  const void *f = kernel_of(tile, prec, akc, bkc, &block, (tile == kDgemm || tile == kDgemm2) ? 1 : 0);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, 0) != hipSuccess || nb < 1) {
     (void)hipGetLastError();
     nb = 1;
@@ -1992,7 +2159,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   const bool dg_ok = (prec == ITX_PREC_F64 || prec == ITX_PREC_F32) && akc && bkc && (vecbits & 1);
   auto finalize = [&](int best, itx_plan **pout) -> int {
     *pout = nullptr;
-    if (best == kDgemm && !dg_ok) return ITX_ERR_UNSUPPORTED;
+    if ((best == kDgemm && !dg_ok) || (best == kDgemm2 && !(dg_ok && prec == ITX_PREC_F64))) return ITX_ERR_UNSUPPORTED;
     const int tbm = kTiles[best].bm, tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
     std::vector<WorkUnit> units, red;
     std::vector<double> ucost;
@@ -2171,7 +2338,9 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     best = kSmall;
   } else if (force) {
     best = std::atoi(force);
-    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow) || (best == kDgemm && !dg_ok)) best = 0;
+    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow) || (best == kDgemm && !dg_ok) ||
+        (best == kDgemm2 && !(dg_ok && prec == ITX_PREC_F64)))
+      best = 0;
   } else if (ctx->tune.count(key)) {
     best = ctx->tune[key];
   } else if (ctx->autotune && !into && !st.empty()) {

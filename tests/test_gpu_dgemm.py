@@ -1,8 +1,9 @@
-"""GPU parity of the register-pipelined tile (k_dgemm, tile 14).
+"""GPU parity of the register-pipelined tiles (k_dgemm, tile 14, and its
+two-workgroups-per-CU variant k_dgemm2, tile 15).
 
-The tile serves FP64 and FP32-form plans whose operands are both contiguous
-along the contracted index in aligned pairs.  It is forced (ITX_FORCE_TILE=14)
-on:
+Tile 14 serves FP64 and FP32-form plans, tile 15 FP64 plans, whose operands
+are both contiguous along the contracted index in aligned pairs.  Each is
+forced (ITX_FORCE_TILE) on:
 
 * plain contractions of every pipeline depth (K = 2 .. 1030: one stage,
   the two- to five-stage prologue/tail paths, and the steady-state loop),
@@ -16,6 +17,8 @@ on:
   k blocks summed into one output tile), against local_op_product;
 * a plan that does not meet the pair condition (odd K): the forced tile is
   refused and a general tile runs instead.
+
+Tile 15 sums in the k order of tile 14: their results are equal bit for bit.
 """
 import numpy as np
 import pytest
@@ -29,6 +32,7 @@ from tests.hpsi_cases import SECTORS_1024, hpsi_operands
 pytestmark = pytest.mark.gpu
 
 TILE = 14
+TILES = [(14, "f64"), (14, "f32"), (15, "f64")]
 SHAPES = [
     # (M, N, K): K / 16 stages = 1, 1, 2, 3, 4, 5, 6, 7, 65
     (1, 1, 2),
@@ -67,29 +71,29 @@ def _plan_run(ctx, dev, A, labA, B, labB, labC, alpha=1.0, beta=0.0, C0=None, sh
     return got, ref, info
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("tile,prec", TILES)
 @pytest.mark.parametrize("shape", SHAPES)
-def test_dgemm_tile_every_depth(ctx, dev, monkeypatch, shape, prec):
-    monkeypatch.setenv("ITX_FORCE_TILE", str(TILE))
+def test_dgemm_tile_every_depth(ctx, dev, monkeypatch, shape, tile, prec):
+    monkeypatch.setenv("ITX_FORCE_TILE", str(tile))
     M, N, K = shape
     rng = np.random.default_rng(9000 + M + N + K)
     A = rng.standard_normal((K, M))
     B = rng.standard_normal((K, N))
     got, ref, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), prec=prec)
-    assert info["tile"] == TILE and info["form"] == prec
+    assert info["tile"] == tile and info["form"] == prec
     assert relerr(got, ref) < TOL[prec], shape
     C0 = rng.standard_normal((M, N))
     got, ref, _ = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), 0.7, -0.3, C0, prec=prec)
     assert relerr(got, ref) < TOL[prec], shape
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("tile,prec", TILES)
 @pytest.mark.parametrize("stage", ["S1", "S4"])
-def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage, prec):
+def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage, tile, prec):
     """The S1 stage (psi x L over one index) and the S4 stage (T3 x L over
     a two-digit index, its digits apart in T3) of the dense H*psi chain at
     chi = 256, with the labels the chain uses."""
-    monkeypatch.setenv("ITX_FORCE_TILE", str(TILE))
+    monkeypatch.setenv("ITX_FORCE_TILE", str(tile))
     chi = 256
     rng = np.random.default_rng(77)
     L = rng.standard_normal((chi, 5, chi))
@@ -99,19 +103,34 @@ def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage, prec):
     else:
         T3 = rng.standard_normal((chi, chi, 2, 2, 5))
         got, ref, info = _plan_run(ctx, dev, T3, (-1, 3, 4, 5, -2), L, (-1, -2, 6), (3, 4, 5, 6), prec=prec)
-    assert info["tile"] == TILE
+    assert info["tile"] == tile
     assert relerr(got, ref) < TOL[prec]
 
 
+@pytest.mark.parametrize("shape", [(129, 257, 48), (513, 129, 1030)])
+def test_dgemm_two_per_cu_equals_tile14(ctx, dev, monkeypatch, shape):
+    M, N, K = shape
+    rng = np.random.default_rng(404 + K)
+    A = rng.standard_normal((K, M))
+    B = rng.standard_normal((K, N))
+    out = {}
+    for tile in (14, 15):
+        monkeypatch.setenv("ITX_FORCE_TILE", str(tile))
+        out[tile], _, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2))
+        assert info["tile"] == tile
+    assert np.array_equal(out[14], out[15])
+
+
+@pytest.mark.parametrize("tile", [14, 15])
 @pytest.mark.parametrize("shape", [(129, 257, 48), (200, 64, 96)])
-def test_dgemm_tile_unaligned_operands(ctx, dev, monkeypatch, shape):
-    monkeypatch.setenv("ITX_FORCE_TILE", str(TILE))
+def test_dgemm_tile_unaligned_operands(ctx, dev, monkeypatch, shape, tile):
+    monkeypatch.setenv("ITX_FORCE_TILE", str(tile))
     M, N, K = shape
     rng = np.random.default_rng(31 + K)
     A = rng.standard_normal((K, M))
     B = rng.standard_normal((K, N))
     got, ref, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), shift=1)
-    assert info["tile"] == TILE
+    assert info["tile"] == tile
     assert relerr(got, ref) < 1e-12
 
 
@@ -132,10 +151,11 @@ def H(torch_dev):
     return host
 
 
-def test_dgemm_tile_block_sparse_hpsi(H, monkeypatch):
+@pytest.mark.parametrize("tile", [14, 15])
+def test_dgemm_tile_block_sparse_hpsi(H, monkeypatch, tile):
     phi, L, W1, W2, R = hpsi_operands(SECTORS_1024, seed=20261202 + 3)
     ref = local_op_product(phi, L, W1, W2, R)
-    monkeypatch.setenv("ITX_FORCE_TILE", str(TILE))
+    monkeypatch.setenv("ITX_FORCE_TILE", str(tile))
     op = H.LocalOp(cten(W1), cten(W2), cten(L), cten(R), True)
     op.set_mode(H.HPSI_PLANNED)
     op.set_precision(0)
