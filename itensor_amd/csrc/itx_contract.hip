@@ -698,19 +698,17 @@ __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ 
       const int64_t *tBK = tab + tk.tBK;
       const int K = tk.K, S = (K + DG_BK - 1) / DG_BK;
       DgRegs ra, rb;
-// load stage st with the k offsets of slot sl (= st & 1, a constant at each
-// use) and fetch stage st + 2's into the same slot
+// load stage st and fetch the k offsets of stage st + 1 (one slot: with two
+// workgroups per CU the table read's wait behind the data loads is covered)
 #define ITX_DG2_LOAD(st, sl)                                  \
   {                                                           \
-    la.load(Ab, ra, sl);                                      \
-    lb.load(Bb, rb, sl);                                      \
-    la.fetch_k(((st) + 2) * DG_BK, K, tAK, sl);               \
-    lb.fetch_k(((st) + 2) * DG_BK, K, tBK, sl);               \
+    la.load(Ab, ra, 0);                                       \
+    lb.load(Bb, rb, 0);                                       \
+    la.fetch_k(((st) + 1) * DG_BK, K, tAK, 0);                \
+    lb.fetch_k(((st) + 1) * DG_BK, K, tBK, 0);                \
   }
       la.fetch_k(0, K, tAK, 0);
       lb.fetch_k(0, K, tBK, 0);
-      la.fetch_k(DG_BK, K, tAK, 1);
-      lb.fetch_k(DG_BK, K, tBK, 1);
       ITX_DG2_LOAD(0, 0)
       __syncthreads();  // the previous task's LDS reads are done
       la.store(As[0], ra);
