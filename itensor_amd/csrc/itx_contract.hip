@@ -633,34 +633,39 @@ __global__ __launch_bounds__(DG_NT) void k_dgemm(const double *__restrict__ A, c
 // holds the next stage.  Two resident workgroups per CU cover each other's
 // barrier waits, epilogues and first loads.  One barrier per stage, before
 // the last slot, which reads the next stage's first slot.
+template <typename T>
 struct DgSlot {  // a lane's operands of one k slot: 4 A and 4 B fragments
-  double a[4], b[4];
+  T a[4], b[4];
 };
 
-__device__ __forceinline__ void dg_read_slot(DgSlot &f, const double *__restrict__ as, const double *__restrict__ bs,
+template <typename T>
+__device__ __forceinline__ void dg_read_slot(DgSlot<T> &f, const T *__restrict__ as, const T *__restrict__ bs,
                                              int wm, int wn, int lane, int s) {
   const int r = lane & 15, q = lane >> 4;
-  constexpr int P = DgCfg<double>::P;
+  constexpr int P = DgCfg<T>::P;
 #pragma unroll
   for (int j = 0; j < 4; ++j) f.a[j] = as[(wm * 64 + j * 16 + r) * P + 4 * q + s];
 #pragma unroll
   for (int i = 0; i < 4; ++i) f.b[i] = bs[(wn * 64 + i * 16 + r) * P + 4 * q + s];
 }
 
-__device__ __forceinline__ void dg_mfma_slot(dv4 (&acc)[4][4], const DgSlot &f) {
+template <typename T>
+__device__ __forceinline__ void dg_mfma_slot(typename MF<T>::v4 (&acc)[4][4], const DgSlot<T> &f) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = MF<double>::mma(f.b[i], f.a[j], acc[i][j]);
+    for (int j = 0; j < 4; ++j) acc[i][j] = MF<T>::mma(f.b[i], f.a[j], acc[i][j]);
 }
 
+template <typename T>
 __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ A, const double *__restrict__ B,
                                                      double *__restrict__ C, const GemmTask *__restrict__ tasks,
                                                      const WorkUnit *__restrict__ units, int nunits,
                                                      const int64_t *__restrict__ tab, double alpha, double beta) {
-  constexpr int P = DgCfg<double>::P;
-  __shared__ __align__(16) double As[2][128 * P];
-  __shared__ __align__(16) double Bs[2][128 * P];
+  using v4 = typename MF<T>::v4;
+  constexpr int P = DgCfg<T>::P;
+  __shared__ __align__(16) T As[2][128 * P];
+  __shared__ __align__(16) T Bs[2][128 * P];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
@@ -680,11 +685,11 @@ __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ 
 
   for (int u = ustart; u < uend; u += ustep) {
     const WorkUnit wu = units[u];
-    dv4 acc[4][4];
+    v4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0, 0, 0, 0};
+      for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
     for (int t = wu.tbeg; t < wu.tend; ++t) {
       const GemmTask tk = tasks[t];
@@ -715,7 +720,7 @@ __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ 
       lb.store(Bs[0], rb);
       if (S > 1) ITX_DG2_LOAD(1, 1)
       __syncthreads();
-      DgSlot cur, nxt;
+      DgSlot<T> cur, nxt;
       dg_read_slot(cur, As[0], Bs[0], wm, wn, lane, 0);
       // stage n (buffer n & 1, load slot sl = n & 1): stage n+1 (in the load
       // registers) into the other buffer, whose stage n-1 reads ended before
@@ -723,7 +728,7 @@ __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ 
       // reading the next slot's operands
 #define ITX_DG2_STAGE(n, sl)                                                      \
   {                                                                               \
-    const double *as = As[sl], *bs = Bs[sl];                                      \
+    const T *as = As[sl], *bs = Bs[sl];                                           \
     if ((n) + 1 < S) {                                                            \
       la.store(As[(sl) ^ 1], ra);                                                 \
       lb.store(Bs[(sl) ^ 1], rb);                                                 \
@@ -765,7 +770,7 @@ __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ 
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = wu.n0 + wn * 64 + i * 16 + MF<double>::orow(lane, r);
+        const int n = wu.n0 + wn * 64 + i * 16 + MF<T>::orow(lane, r);
         nv[i][r] = n < t0.N;
         cn[i][r] = tCN[nv[i][r] ? n : 0];
       }
@@ -777,7 +782,7 @@ __global__ __launch_bounds__(DG_NT, 2) void k_dgemm2(const double *__restrict__ 
         for (int r = 0; r < 4; ++r) {
           if (mv[j] && nv[i][r]) {
             const int64_t ad = cm[j] + cn[i][r];
-            double v = alpha * acc[i][j][r];
+            double v = alpha * double(acc[i][j][r]);
             if (beta != 0.0) v += beta * Cb[ad];
             Cb[ad] = v;
           }
@@ -1723,7 +1728,7 @@ constexpr TileCfg kTiles[] = {
     {128, 64, 2, 2, 0.90, 0.95},   // 12: as 1 with 32-deep k stages
     {64, 128, 2, 2, 0.90, 0.95},   // 13: as 2 with 32-deep k stages
     {128, 128, 2, 2, 1.15, 0.85},  // 14: register-pipelined (k_dgemm), k-contiguous pairs only
-    {128, 128, 2, 2, 1.10, 0.85},  // 15: as 14, two workgroups per CU (k_dgemm2), FP64 only
+    {128, 128, 2, 2, 1.10, 0.85},  // 15: as 14, two workgroups per CU (k_dgemm2)
 };
 constexpr int kNumTiles = 16;
 constexpr int kDgemm = 14, kDgemm2 = 15;
@@ -1796,7 +1801,7 @@ const void *gemm_fn_tile(int tile, bool akc, bool bkc, bool vec) {
       if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm<T>);
       return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, false);
     case 15:
-      if (std::is_same<T, double>::value && akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm2);
+      if (akc && bkc && vec) return reinterpret_cast<const void *>(&k_dgemm2<T>);
       return gemm_fn<T, 128, 128, 2, 2>(akc, bkc, false);
     default: return gemm_fn<T, 32, 32, 2, 2>(akc, bkc, vec);
   }
@@ -2157,7 +2162,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   const bool dg_ok = (prec == ITX_PREC_F64 || prec == ITX_PREC_F32) && akc && bkc && (vecbits & 1);
   auto finalize = [&](int best, itx_plan **pout) -> int {
     *pout = nullptr;
-    if ((best == kDgemm && !dg_ok) || (best == kDgemm2 && !(dg_ok && prec == ITX_PREC_F64))) return ITX_ERR_UNSUPPORTED;
+    if ((best == kDgemm || best == kDgemm2) && !dg_ok) return ITX_ERR_UNSUPPORTED;
     const int tbm = kTiles[best].bm, tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
     std::vector<WorkUnit> units, red;
     std::vector<double> ucost;
@@ -2336,8 +2341,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     best = kSmall;
   } else if (force) {
     best = std::atoi(force);
-    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow) || (best == kDgemm && !dg_ok) ||
-        (best == kDgemm2 && !(dg_ok && prec == ITX_PREC_F64)))
+    if (best < 0 || best >= kNumTiles || (best == kSkinny && !narrow) || ((best == kDgemm || best == kDgemm2) && !dg_ok))
       best = 0;
   } else if (ctx->tune.count(key)) {
     best = ctx->tune[key];

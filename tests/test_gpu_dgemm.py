@@ -1,9 +1,9 @@
 """GPU parity of the register-pipelined tiles (k_dgemm, tile 14, and its
 two-workgroups-per-CU variant k_dgemm2, tile 15).
 
-Tile 14 serves FP64 and FP32-form plans, tile 15 FP64 plans, whose operands
-are both contiguous along the contracted index in aligned pairs.  Each is
-forced (ITX_FORCE_TILE) on:
+Both serve FP64 and FP32-form plans whose operands are both contiguous
+along the contracted index in aligned pairs.  Each is forced (ITX_FORCE_TILE)
+on:
 
 * plain contractions of every pipeline depth (K = 2 .. 1030: one stage,
   the two- to five-stage prologue/tail paths, and the steady-state loop),
@@ -32,7 +32,7 @@ from tests.hpsi_cases import SECTORS_1024, hpsi_operands
 pytestmark = pytest.mark.gpu
 
 TILE = 14
-TILES = [(14, "f64"), (14, "f32"), (15, "f64")]
+TILES = [(14, "f64"), (14, "f32"), (15, "f64"), (15, "f32")]
 SHAPES = [
     # (M, N, K): K / 16 stages = 1, 1, 2, 3, 4, 5, 6, 7, 65
     (1, 1, 2),
@@ -107,8 +107,9 @@ def test_dgemm_tile_hpsi_stages(ctx, dev, monkeypatch, stage, tile, prec):
     assert relerr(got, ref) < TOL[prec]
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("shape", [(129, 257, 48), (513, 129, 1030)])
-def test_dgemm_two_per_cu_equals_tile14(ctx, dev, monkeypatch, shape):
+def test_dgemm_two_per_cu_equals_tile14(ctx, dev, monkeypatch, shape, prec):
     M, N, K = shape
     rng = np.random.default_rng(404 + K)
     A = rng.standard_normal((K, M))
@@ -116,7 +117,7 @@ def test_dgemm_two_per_cu_equals_tile14(ctx, dev, monkeypatch, shape):
     out = {}
     for tile in (14, 15):
         monkeypatch.setenv("ITX_FORCE_TILE", str(tile))
-        out[tile], _, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2))
+        out[tile], _, info = _plan_run(ctx, dev, A, (-1, 1), B, (-1, 2), (1, 2), prec=prec)
         assert info["tile"] == tile
     assert np.array_equal(out[14], out[15])
 
