@@ -1161,7 +1161,10 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny(const double *__restrict__ A
 // as fit are staged in LDS at once (rows stacked), and the A elements of
 // SKQ_G tasks are loaded together before any is used.  Same accumulation
 // order as k_skinny (tasks in order, k in order), so the same results.
-constexpr int SKQ_K = 8, SKQ_N = 8, SKQ_G = 4;
+// Each thread computes two rows, SK_ROWS apart (units of 2 * SK_ROWS rows):
+// the per-unit task reads, B staging and barriers are shared by twice the
+// rows, and a thread has the loads of two rows in flight.
+constexpr int SKQ_K = 8, SKQ_N = 8, SKQ_G = 2, SKQ_R = 2;
 
 __global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__ A, const double *__restrict__ B,
                                                       double *__restrict__ C, const GemmTask *__restrict__ tasks,
@@ -1173,10 +1176,18 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__
     const WorkUnit wu = units[u];
     const GemmTask t0 = tasks[wu.tbeg];
     const int N = t0.N, M = t0.M;
-    const int m = wu.m0 + tid;
-    double acc[SKQ_N];
+    int mr[SKQ_R];
+    bool in[SKQ_R];
 #pragma unroll
-    for (int n = 0; n < SKQ_N; ++n) acc[n] = 0.0;
+    for (int h = 0; h < SKQ_R; ++h) {
+      mr[h] = wu.m0 + tid + h * SK_ROWS;
+      in[h] = mr[h] < M;
+    }
+    double acc[SKQ_R][SKQ_N];
+#pragma unroll
+    for (int h = 0; h < SKQ_R; ++h)
+#pragma unroll
+      for (int n = 0; n < SKQ_N; ++n) acc[h][n] = 0.0;
     for (int tc = wu.tbeg; tc < wu.tend;) {
       // the next tasks whose stacked B rows fit the LDS tile
       int te = tc, kt = 0;
@@ -1191,9 +1202,9 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__
         kb += tk.K;
       }
       __syncthreads();
-      if (m < M) {
+      if (in[0]) {
         for (int t = tc, kb = 0; t < te; t += SKQ_G) {
-          double av[SKQ_G][SKQ_K];
+          double av[SKQ_R][SKQ_G][SKQ_K];
           int kk[SKQ_G];
 #pragma unroll
           for (int q = 0; q < SKQ_G; ++q) {
@@ -1201,10 +1212,13 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__
             if (t + q < te) {
               const GemmTask tk = tasks[t + q];
               kk[q] = tk.K;
-              const double *Ar = A + tk.a_off + tab[tk.tAM + m];
               const int64_t *tAK = tab + tk.tAK;
 #pragma unroll
-              for (int j = 0; j < SKQ_K; ++j) av[q][j] = (j < tk.K) ? Ar[tAK[j]] : 0.0;
+              for (int h = 0; h < SKQ_R; ++h) {
+                const double *Ar = A + tk.a_off + tab[tk.tAM + (in[h] ? mr[h] : mr[0])];
+#pragma unroll
+                for (int j = 0; j < SKQ_K; ++j) av[h][q][j] = (j < tk.K && in[h]) ? Ar[tAK[j]] : 0.0;
+              }
             }
           }
 #pragma unroll
@@ -1214,7 +1228,11 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__
               if (j < kk[q]) {
 #pragma unroll
                 for (int n = 0; n < SKQ_N; ++n)
-                  if (n < N) acc[n] += av[q][j] * Bs[(kb + j) * SKQ_N + n];
+                  if (n < N) {
+                    const double bv = Bs[(kb + j) * SKQ_N + n];
+#pragma unroll
+                    for (int h = 0; h < SKQ_R; ++h) acc[h][n] += av[h][q][j] * bv;
+                  }
               }
             }
             kb += kk[q];
@@ -1223,15 +1241,18 @@ __global__ __launch_bounds__(SK_ROWS) void k_skinny_q(const double *__restrict__
       }
       tc = te;
     }
-    if (m < M) {
-      double *Cm = C + t0.c_off + tab[t0.tCM + m];
-      const int64_t *tCN = tab + t0.tCN;
 #pragma unroll
-      for (int n = 0; n < SKQ_N; ++n) {
-        if (n < N) {
-          double v = alpha * acc[n];
-          if (beta != 0.0) v += beta * Cm[tCN[n]];
-          Cm[tCN[n]] = v;
+    for (int h = 0; h < SKQ_R; ++h) {
+      if (in[h]) {
+        double *Cm = C + t0.c_off + tab[t0.tCM + mr[h]];
+        const int64_t *tCN = tab + t0.tCN;
+#pragma unroll
+        for (int n = 0; n < SKQ_N; ++n) {
+          if (n < N) {
+            double v = alpha * acc[h][n];
+            if (beta != 0.0) v += beta * Cm[tCN[n]];
+            Cm[tCN[n]] = v;
+          }
         }
       }
     }
@@ -2160,10 +2181,13 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   // Build the device plan for one tile choice.
   // the register-pipelined FP64 tile needs k-contiguous operands in aligned pairs
   const bool dg_ok = (prec == ITX_PREC_F64 || prec == ITX_PREC_F32) && akc && bkc && (vecbits & 1);
+  // skinny plans whose tasks all have K, N <= 8 run k_skinny_q: two rows per thread
+  const bool tinyq = maxN <= SKQ_N && maxK <= SKQ_K;
   auto finalize = [&](int best, itx_plan **pout) -> int {
     *pout = nullptr;
     if ((best == kDgemm || best == kDgemm2) && !dg_ok) return ITX_ERR_UNSUPPORTED;
-    const int tbm = kTiles[best].bm, tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
+    const int tbm = (best == kSkinny && tinyq) ? SKQ_R * SK_ROWS : kTiles[best].bm;
+    const int tbn = (best == kSkinny) ? std::max(1, maxN) : kTiles[best].bn;
     std::vector<WorkUnit> units, red;
     std::vector<double> ucost;
     std::vector<GemmTask> vt;  // device task list (st, or its k-chunks when split)
@@ -2255,7 +2279,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     p->flops = flops;
     p->grid = int(std::min<int64_t>(int64_t(units.size()), int64_t(cus) * tile_occupancy(best, prec, akc, bkc)));
     p->vec = (akc && bkc) ? vecbits : akc ? (vecbits & 8) : 0;
-    const bool tiny = best == kSkinny && maxN <= SKQ_N && maxK <= SKQ_K;
+    const bool tiny = best == kSkinny && tinyq;
     p->fn = kernel_of(best, prec, akc, bkc, &p->block, p->vec, false, tiny);
     p->fn_scalar = kernel_of(best, prec, akc, bkc, &p->block, 0, false, tiny);
     p->nred = int(red.size());
