@@ -2300,11 +2300,12 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
       size_t o_tab = (o_red + br_bytes + 255) & ~size_t(255);
       size_t o_ws = (o_tab + tab.size() * sizeof(int64_t) + 255) & ~size_t(255);
       size_t total = o_ws + sizeof(double) * size_t(nslots) * (SM_M * SM_N);
-      std::vector<char> host(o_ws, 0);
-      std::memcpy(host.data(), vt.data(), bt_bytes);
-      std::memcpy(host.data() + o_units, units.data(), bu_bytes);
-      if (br_bytes) std::memcpy(host.data() + o_red, red.data(), br_bytes);
-      if (!tab.empty()) std::memcpy(host.data() + o_tab, tab.data(), tab.size() * sizeof(int64_t));
+      // the device image: tasks | units | reduction units | tables, each
+      // 256-byte aligned, assembled in the staging arena
+      const UploadPart parts[4] = {{vt.data(), 0, bt_bytes},
+                                   {units.data(), o_units, bu_bytes},
+                                   {red.data(), o_red, br_bytes},
+                                   {tab.data(), o_tab, tab.size() * sizeof(int64_t)}};
       int rc3 = ITX_OK;
       if (into) {
         if (total > into_cap) {
@@ -2323,7 +2324,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
         }
       }
       p->ctx = ctx;
-      rc3 = stage_upload(ctx, p->dev, host.data(), o_ws);
+      rc3 = stage_upload_parts(ctx, p->dev, parts, 4, o_ws);
       if (rc3) {
         if (p->own_dev) pool_free(ctx, p->dev);
         delete p;

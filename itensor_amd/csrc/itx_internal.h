@@ -120,6 +120,12 @@ struct itx_ctx {
   uint64_t hres_seq = 0;  // completion flag value of the last host reduction
   char *stage = nullptr;
   size_t stage_cap = 0, stage_off = 0;
+  // the arena is used as two halves; stage_ev[h] is recorded on the stream
+  // when the uploads move out of half h (stage_ev_used[h]: recorded at least
+  // once), so that reusing a half waits only for the copies issued from it
+  int stage_half = 0;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_ev_used[2] = {false, false};
   // plan cache for the one-shot contraction entry points
   std::unordered_map<std::string, itx_plan *> plan_cache;
   std::vector<std::string> plan_lru;
@@ -143,10 +149,18 @@ int hip_err(itx_ctx *ctx, hipError_t e, const char *where);
 int pool_alloc(itx_ctx *ctx, size_t bytes, void **p);
 int pool_free(itx_ctx *ctx, void *p);
 // Host-to-device copy ordered on the context's stream that returns without
-// waiting: the bytes go through the pinned staging arena (the stream is
-// synchronised only when the arena wraps around; copies larger than half
-// the arena are synchronous).
+// waiting: the bytes go through the pinned staging arena (moving into one of
+// its halves waits for the copies last issued from that half; copies larger
+// than half the arena are synchronous).
 int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes);
+// One upload of `total` bytes assembled from pieces (source, offset in the
+// image, bytes) directly in the staging arena: no intermediate host image.
+// Bytes of the image that no piece covers are left unspecified.
+struct UploadPart {
+  const void *src;
+  size_t off, bytes;
+};
+int stage_upload_parts(itx_ctx *ctx, void *dst, const UploadPart *parts, int nparts, size_t total);
 constexpr int HRES_SLOTS = 64;  // pinned result slots (multi-dot up to 16)
 
 // Build a contraction plan (dense = one task).  Shared by the dense and

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <mutex>
 
+#include <vector>
 #include "itx_internal.h"
 
 namespace {
@@ -65,6 +66,29 @@ int pool_free(itx_ctx *ctx, void *p) {
   return ITX_OK;
 }
 
+// Offset in the arena for an upload of `bytes` (at most half the arena):
+// the next 256-byte aligned place in the current half, or the start of the
+// other half once that half's last copies are complete.
+static int stage_reserve(itx_ctx *ctx, size_t bytes, size_t *out) {
+  const size_t half = ctx->stage_cap / 2;
+  size_t off = (ctx->stage_off + 255) & ~size_t(255);
+  if (off + bytes > size_t(ctx->stage_half) * half + half) {
+    const int h = ctx->stage_half, nh = h ^ 1;
+    if (ctx->stage_ev[0] && ctx->stage_ev[1]) {
+      ITX_CHECK_HIP(ctx, hipEventRecord(ctx->stage_ev[h], ctx->stream));
+      ctx->stage_ev_used[h] = true;
+      if (ctx->stage_ev_used[nh]) ITX_CHECK_HIP(ctx, hipEventSynchronize(ctx->stage_ev[nh]));
+    } else {
+      // every earlier copy out of the arena is complete once the stream is
+      ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    ctx->stage_half = nh;
+    off = size_t(nh) * half;
+  }
+  *out = off;
+  return ITX_OK;
+}
+
 int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (bytes == 0) return ITX_OK;
   if (!ctx->stage || bytes > ctx->stage_cap / 2) {
@@ -72,15 +96,30 @@ int stage_upload(itx_ctx *ctx, void *dst, const void *src, size_t bytes) {
     ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ITX_OK;
   }
-  size_t off = (ctx->stage_off + 255) & ~size_t(255);
-  if (off + bytes > ctx->stage_cap) {
-    // every earlier copy out of the arena is complete once the stream is
-    ITX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    off = 0;
-  }
+  size_t off = 0;
+  int rc = stage_reserve(ctx, bytes, &off);
+  if (rc) return rc;
   std::memcpy(ctx->stage + off, src, bytes);
   ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, ctx->stage + off, bytes, hipMemcpyHostToDevice, ctx->stream));
   ctx->stage_off = off + bytes;
+  return ITX_OK;
+}
+
+int stage_upload_parts(itx_ctx *ctx, void *dst, const UploadPart *parts, int nparts, size_t total) {
+  if (total == 0) return ITX_OK;
+  if (!ctx->stage || total > ctx->stage_cap / 2) {
+    std::vector<char> img(total);
+    for (int i = 0; i < nparts; ++i)
+      if (parts[i].bytes) std::memcpy(img.data() + parts[i].off, parts[i].src, parts[i].bytes);
+    return stage_upload(ctx, dst, img.data(), total);
+  }
+  size_t off = 0;
+  int rc = stage_reserve(ctx, total, &off);
+  if (rc) return rc;
+  for (int i = 0; i < nparts; ++i)
+    if (parts[i].bytes) std::memcpy(ctx->stage + off + parts[i].off, parts[i].src, parts[i].bytes);
+  ITX_CHECK_HIP(ctx, hipMemcpyAsync(dst, ctx->stage + off, total, hipMemcpyHostToDevice, ctx->stream));
+  ctx->stage_off = off + total;
   return ITX_OK;
 }
 
@@ -162,11 +201,20 @@ int itx_create(int device, itx_ctx **out) {
   ctx->hres = static_cast<double *>(hp);
   ctx->hres_dev = static_cast<double *>(hd);
   ctx->stage_cap = kStageBytes;
+  // without the events the arena falls back to a stream synchronisation
+  if (hipEventCreateWithFlags(&ctx->stage_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->stage_ev[1], hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (ctx->stage_ev[0]) hipEventDestroy(ctx->stage_ev[0]);
+    ctx->stage_ev[0] = ctx->stage_ev[1] = nullptr;
+  }
   // this device's copy of the static scratch array (the device is current)
   void *sp = nullptr;
   e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
   if (e != hipSuccess || !sp) {
     g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    for (auto &ev : ctx->stage_ev)
+      if (ev) hipEventDestroy(ev);
     hipHostFree(ctx->stage);
     hipHostFree(hp);
     hipFree(ctx->red);
@@ -199,6 +247,8 @@ int itx_destroy(itx_ctx *ctx) {
   for (auto &kv : ctx->host_live) hipHostFree(kv.first);
   if (ctx->hres) hipHostFree(ctx->hres);
   if (ctx->stage) hipHostFree(ctx->stage);
+  for (auto &ev : ctx->stage_ev)
+    if (ev) hipEventDestroy(ev);
   delete ctx;
   return ITX_OK;
 }
