@@ -1616,12 +1616,22 @@ int64_t emit_tables(const Digits &d, std::vector<int64_t> &tab) {
   tab.resize(tab.size() + 2 * size_t(n));
   int64_t *t1 = tab.data() + start, *t2 = t1 + n;
   const int r = d.size();
+  if (n == 0) return start;
+  if (r == 0) {  // a scalar: one entry
+    t1[0] = t2[0] = 0;
+    return start;
+  }
+  // first digit fastest: runs of d[0].dim entries in arithmetic progression,
+  // the other digits advanced once per run
+  const int64_t d0 = d[0].dim, a0 = d[0].s1, b0 = d[0].s2;
   int64_t ctr[ITX_MAX_RANK] = {};
   int64_t o1 = 0, o2 = 0;
-  for (int64_t x = 0; x < n; ++x) {
-    t1[x] = o1;
-    t2[x] = o2;
-    for (int i = 0; i < r; ++i) {  // first digit fastest
+  for (int64_t x = 0; x < n; x += d0) {
+    for (int64_t i = 0; i < d0; ++i) {
+      t1[x + i] = o1 + i * a0;
+      t2[x + i] = o2 + i * b0;
+    }
+    for (int i = 1; i < r; ++i) {
       ctr[i]++;
       o1 += d[i].s1;
       o2 += d[i].s2;
