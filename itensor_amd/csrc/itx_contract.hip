@@ -2033,30 +2033,51 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     // consecutive k one run of V consecutive elements starting at an even
     // offset, every row offset even (16-byte aligned given an aligned base)
     fi.vec = 0;
-    // runs of V consecutive k, each starting at a multiple of `al`
-    auto runs = [&](int64_t t, int64_t n, int V, int al) {
-      if (n % V) return false;
-      for (int64_t k = 0; k < n; k += V) {
-        if (tab[size_t(t + k)] % al) return false;
-        for (int j = 1; j < V; ++j)
-          if (tab[size_t(t + k + j)] != tab[size_t(t + k)] + j) return false;
+    // Decided from the digits instead of scanning the tables: one column of
+    // a digit set (s1 or s2) re-merged where it is contiguous by itself;
+    // every offset of that table is a multiple of `al` iff every stride is,
+    // and the table comes in runs of V consecutive offsets starting at
+    // multiples of `al` iff V divides its size, the fastest digit has
+    // stride 1 and a multiple of V entries, and the other strides are
+    // multiples of `al` (checked against the table scan on random digit
+    // sets, both columns, V = 2, 4 and al = 2, 4)
+    auto column = [](const Digits &d, bool first, Digit *c) {
+      int nc = 0;
+      for (auto &x : d) {
+        if (x.dim == 1) continue;
+        const int64_t st = first ? x.s1 : x.s2;
+        if (nc > 0 && c[nc - 1].s1 * c[nc - 1].dim == st)
+          c[nc - 1].dim *= x.dim;
+        else
+          c[nc++] = Digit{x.dim, st, 0};
       }
+      return nc;
+    };
+    auto aligned = [&](const Digits &d, bool first, int al) {
+      Digit c[ITX_MAX_RANK];
+      const int nc = column(d, first, c);
+      for (int i = 0; i < nc; ++i)
+        if (c[i].s1 % al) return false;
       return true;
     };
-    auto aligned = [&](int64_t t, int64_t n, int al) {
-      for (int64_t i = 0; i < n; ++i)
-        if (tab[size_t(t + i)] % al) return false;
+    auto runs = [&](const Digits &d, bool first, int64_t n, int V, int al) {
+      if (n % V) return false;
+      Digit c[ITX_MAX_RANK];
+      const int nc = column(d, first, c);
+      if (nc == 0 || c[0].s1 != 1 || c[0].dim % V) return false;
+      for (int i = 1; i < nc; ++i)
+        if (c[i].s1 % al) return false;
       return true;
     };
     // bit 3: A alone in runs of 4 at even offsets (the skinny path reads
     // only A from memory)
-    if (fi.akc && aligned(fi.tAM, M, 2) && runs(fi.tAK, K, 4, 2)) fi.vec |= 8;
+    if (fi.akc && aligned(dm, true, 2) && runs(dk, true, K, 4, 2)) fi.vec |= 8;
     if (fi.akc && fi.bkc) {
-      if (aligned(fi.tAM, M, 2) && aligned(fi.tBN, N, 2)) {
-        if (runs(fi.tAK, K, 2, 2) && runs(fi.tBK, K, 2, 2)) fi.vec |= 1;
-        if (runs(fi.tAK, K, 4, 2) && runs(fi.tBK, K, 4, 2)) fi.vec |= 2;
+      if (aligned(dm, true, 2) && aligned(dn, true, 2)) {
+        if (runs(dk, true, K, 2, 2) && runs(dk, false, K, 2, 2)) fi.vec |= 1;
+        if (runs(dk, true, K, 4, 2) && runs(dk, false, K, 4, 2)) fi.vec |= 2;
       }
-      if (aligned(fi.tAM, M, 4) && aligned(fi.tBN, N, 4) && runs(fi.tAK, K, 4, 4) && runs(fi.tBK, K, 4, 4))
+      if (aligned(dm, true, 4) && aligned(dn, true, 4) && runs(dk, true, K, 4, 4) && runs(dk, false, K, 4, 4))
         fi.vec |= 4;
     }
     return ITX_OK;
