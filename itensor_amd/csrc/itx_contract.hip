@@ -1608,9 +1608,32 @@ int64_t digits_size(const Digits &d) {
   return s;
 }
 
+// Offset tables of a plan: every entry is written by emit_tables, so growing
+// the vector leaves new entries uninitialised (default- instead of value-
+// initialised: no zeroing pass over the megabytes of a large QN plan).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <class U>
+  void construct(U *p) noexcept {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <class U, class... Args>
+  void construct(U *p, Args &&...args) {
+    ::new (static_cast<void *>(p)) U(std::forward<Args>(args)...);
+  }
+};
+using TabVec = std::vector<int64_t, NoInitAlloc<int64_t>>;
+
 // Append the two offset tables of a digit set to tab; returns the start of
 // the first table (the second starts size() later).
-int64_t emit_tables(const Digits &d, std::vector<int64_t> &tab) {
+int64_t emit_tables(const Digits &d, TabVec &tab) {
   const int64_t n = digits_size(d);
   const int64_t start = int64_t(tab.size());
   tab.resize(tab.size() + 2 * size_t(n));
@@ -1661,7 +1684,7 @@ struct TableCache {
     }
   };
   std::map<Key, int64_t> at;
-  int64_t emit(const Digits &d, std::vector<int64_t> &tab) {
+  int64_t emit(const Digits &d, TabVec &tab) {
     const Key k{d};
     auto it = at.find(k);
     if (it != at.end()) return it->second;
@@ -1947,7 +1970,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
   if (rc) return rc;
 
   std::vector<GemmTask> tasks;
-  std::vector<int64_t> tab;
+  TabVec tab;
   TableCache tcache;
   tasks.reserve(ntasks);
   // operand extents (elements) for the autotuner's scratch buffers
