@@ -1608,7 +1608,7 @@ int64_t digits_size(const Digits &d) {
   return s;
 }
 
-// Offset tables of a plan: every entry is written by emit_tables, so growing
+// Offset tables of a plan: every entry is written by ColumnCache::emit, so growing
 // the vector leaves new entries uninitialised (default- instead of value-
 // initialised: no zeroing pass over the megabytes of a large QN plan).
 template <class T>
@@ -1631,66 +1631,83 @@ struct NoInitAlloc : std::allocator<T> {
 };
 using TabVec = std::vector<int64_t, NoInitAlloc<int64_t>>;
 
-// Append the two offset tables of a digit set to tab; returns the start of
-// the first table (the second starts size() later).
-int64_t emit_tables(const Digits &d, TabVec &tab) {
-  const int64_t n = digits_size(d);
-  const int64_t start = int64_t(tab.size());
-  tab.resize(tab.size() + 2 * size_t(n));
-  int64_t *t1 = tab.data() + start, *t2 = t1 + n;
-  const int r = d.size();
-  if (n == 0) return start;
-  if (r == 0) {  // a scalar: one entry
-    t1[0] = t2[0] = 0;
-    return start;
+// Offset tables are emitted one column at a time (the A / C row offsets of
+// the m digits, B / C of the n digits, A / B of the k digits), each from
+// the column's own digit list: the digits with more than one value, merged
+// where the column is contiguous by itself (stride of one = dim x stride of
+// the previous).  The merged list enumerates the same offsets in the same
+// order as the joint digits (first digit fastest) and determines the table,
+// so equal columns share one table, and a column that is one run of stride
+// 1 (most block rows and k runs) is a prefix of a single 0, 1, 2, ... table.
+struct ColDigit {
+  int64_t dim, s;
+};
+int column_digits(const Digits &d, bool first, ColDigit *c) {
+  int nc = 0;
+  for (auto &x : d) {
+    if (x.dim == 1) continue;
+    const int64_t st = first ? x.s1 : x.s2;
+    if (nc > 0 && c[nc - 1].s * c[nc - 1].dim == st)
+      c[nc - 1].dim *= x.dim;
+    else
+      c[nc++] = ColDigit{x.dim, st};
   }
-  // first digit fastest: runs of d[0].dim entries in arithmetic progression,
-  // the other digits advanced once per run
-  const int64_t d0 = d[0].dim, a0 = d[0].s1, b0 = d[0].s2;
-  int64_t ctr[ITX_MAX_RANK] = {};
-  int64_t o1 = 0, o2 = 0;
-  for (int64_t x = 0; x < n; x += d0) {
-    for (int64_t i = 0; i < d0; ++i) {
-      t1[x + i] = o1 + i * a0;
-      t2[x + i] = o2 + i * b0;
-    }
-    for (int i = 1; i < r; ++i) {
-      ctr[i]++;
-      o1 += d[i].s1;
-      o2 += d[i].s2;
-      if (ctr[i] < d[i].dim) break;
-      o1 -= d[i].s1 * d[i].dim;
-      o2 -= d[i].s2 * d[i].dim;
-      ctr[i] = 0;
-    }
-  }
-  return start;
+  return nc;
 }
 
-// emit_tables through a cache: blocks of equal shape and strides (most
-// of a QN task list) share one pair of tables
-struct TableCache {
+struct ColumnCache {
   struct Key {
-    Digits d;
+    int n;
+    ColDigit c[ITX_MAX_RANK];
     bool operator<(const Key &o) const {
-      if (d.n != o.d.n) return d.n < o.d.n;
-      for (int i = 0; i < d.n; ++i) {
-        const Digit &x = d.d[i], &y = o.d.d[i];
-        if (x.dim != y.dim) return x.dim < y.dim;
-        if (x.s1 != y.s1) return x.s1 < y.s1;
-        if (x.s2 != y.s2) return x.s2 < y.s2;
+      if (n != o.n) return n < o.n;
+      for (int i = 0; i < n; ++i) {
+        if (c[i].dim != o.c[i].dim) return c[i].dim < o.c[i].dim;
+        if (c[i].s != o.c[i].s) return c[i].s < o.c[i].s;
       }
       return false;
     }
   };
   std::map<Key, int64_t> at;
-  int64_t emit(const Digits &d, TabVec &tab) {
-    const Key k{d};
+  int64_t iota_start = -1, iota_len = 0;
+  // start of the table of column `first` (s1) or not (s2) of d in tab
+  int64_t emit(const Digits &d, bool first, TabVec &tab) {
+    Key k;
+    k.n = column_digits(d, first, k.c);
+    int64_t n = 1;
+    for (int i = 0; i < k.n; ++i) n *= k.c[i].dim;
+    if (n == 0) return int64_t(tab.size());  // an empty extent reads nothing
+    if (k.n == 0 || (k.n == 1 && k.c[0].s == 1)) {
+      if (iota_len < n) {  // a longer 0, 1, 2, ... table (earlier ones stay valid)
+        iota_len = std::max<int64_t>(n, 2 * iota_len);
+        iota_start = int64_t(tab.size());
+        tab.resize(tab.size() + size_t(iota_len));
+        int64_t *t = tab.data() + iota_start;
+        for (int64_t x = 0; x < iota_len; ++x) t[x] = x;
+      }
+      return iota_start;
+    }
     auto it = at.find(k);
     if (it != at.end()) return it->second;
-    const int64_t s = emit_tables(d, tab);
-    at.emplace(k, s);
-    return s;
+    const int64_t start = int64_t(tab.size());
+    tab.resize(tab.size() + size_t(n));
+    int64_t *t = tab.data() + start;
+    // runs along the fastest digit, the other digits advanced once per run
+    const int64_t d0 = k.c[0].dim, a0 = k.c[0].s;
+    int64_t ctr[ITX_MAX_RANK] = {};
+    int64_t o = 0;
+    for (int64_t x = 0; x < n; x += d0) {
+      for (int64_t i = 0; i < d0; ++i) t[x + i] = o + i * a0;
+      for (int i = 1; i < k.n; ++i) {
+        ctr[i]++;
+        o += k.c[i].s;
+        if (ctr[i] < k.c[i].dim) break;
+        o -= k.c[i].s * k.c[i].dim;
+        ctr[i] = 0;
+      }
+    }
+    at.emplace(k, start);
+    return start;
   }
 };
 // largest offset either table of a digit set holds (strides are positive)
@@ -1971,7 +1988,7 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
 
   std::vector<GemmTask> tasks;
   TabVec tab;
-  TableCache tcache;
+  ColumnCache ccache;
   tasks.reserve(ntasks);
   // operand extents (elements) for the autotuner's scratch buffers
   int64_t extA = 1, extB = 1, extC = 1;
@@ -2040,15 +2057,12 @@ int build_plan(itx_ctx *ctx, int rA, const int32_t *labA, int rB, const int32_t 
     }
     fi.akc = !dk.empty() && dk[0].s1 == 1;
     fi.bkc = !dk.empty() && dk[0].s2 == 1;
-    int64_t t0 = tcache.emit(dm, tab);
-    fi.tAM = t0;
-    fi.tCM = t0 + M;
-    t0 = tcache.emit(dn, tab);
-    fi.tBN = t0;
-    fi.tCN = t0 + N;
-    t0 = tcache.emit(dk, tab);
-    fi.tAK = t0;
-    fi.tBK = t0 + K;
+    fi.tAM = ccache.emit(dm, true, tab);
+    fi.tCM = ccache.emit(dm, false, tab);
+    fi.tBN = ccache.emit(dn, true, tab);
+    fi.tCN = ccache.emit(dn, false, tab);
+    fi.tAK = ccache.emit(dk, true, tab);
+    fi.tBK = ccache.emit(dk, false, tab);
     digits_max(dm, fi.am, fi.cm);
     digits_max(dn, fi.bn, fi.cn);
     digits_max(dk, fi.ak, fi.bk);
